@@ -1,0 +1,133 @@
+/*
+ * qap_pt.h -- C-ABI of the MI355X parallel-tempering QAP annealer.
+ *
+ * Drop-in for the reference's PT-SA solve path:
+ *   CPU-PIM/CPU_PIM.cpp:478-685   main(): readProblem -> setup -> threadToRun x T
+ *   GPU-PIM/GPU_PIM.cu:742-1057   main(): setup -> persistent_sim_kernel launch
+ *   GPU-CUPIM/perl-gpu/src/lib/qap/qap.cu:28-303  qap::ReplicaPool<F,Z> (staged pool API)
+ * The reference exposes no library API (state lives in globals, inputs come from a
+ * file and flags, results go to stdout/log.txt).  These entry points carry exactly
+ * what its flags and globals carry:
+ *   -b beta0 / -m ratio  (CPU_PIM.cpp:526-561)  -> qap_pt_params.beta_ln (explicit ladder)
+ *   -r replicas          (CPU_PIM.cpp:569-580)  -> qap_pt_params.R
+ *   -t threads (ladders) (CPU_PIM.cpp:588-599)  -> qap_pt_params.T
+ *   -w megas, -i NUMITERA(CPU_PIM.cpp:548-606)  -> sweeps = megas * NUMITERA, sweeps_per_exchange
+ *   -g goal, -s timeout  (CPU_PIM.cpp:534-587)  -> has_goal/goal, timeout_s
+ *   "Single best energy found" (CPU_PIM.cpp:678) -> qap_pt_result.best_cost (+ best_perm,
+ *                                                 which the reference only prints with -G)
+ *
+ * Conventions: every call returns QAP_PT_OK (0) or a negative status and leaves a
+ * message for qap_pt_last_error() (thread-local).  The caller owns every buffer;
+ * inputs are only read during the call.  Solve calls are reentrant; one pool or
+ * server handle must not be used from two threads at once.  No C++ exception
+ * crosses this boundary.  All costs are exact integers (int64); force matrices are
+ * int32 and a permutation-independent bound is checked before any launch
+ * (QAP_PT_E_OVERFLOW when it cannot be proven).
+ */
+#ifndef QAP_PT_H
+#define QAP_PT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QAP_PT_OK 0
+#define QAP_PT_E_INVAL (-1)
+#define QAP_PT_E_NOMEM (-2)
+#define QAP_PT_E_DEVICE (-3)
+#define QAP_PT_E_OVERFLOW (-4)
+#define QAP_PT_E_BUSY (-5)
+
+/* Largest problem size the sweep kernels accept. */
+#define QAP_PT_MAX_N 1024
+
+/* qap_pt_params.flags */
+#define QAP_PT_INIT_PER_REPLICA 1u /* own initial shuffle per replica (GPU_PIM.cu:151-167)
+                                      instead of one per ladder (CPU_PIM.cpp:143-161)     */
+
+/* One instance: n, then flow and distance, both n*n row-major int32, exactly as
+ * readProblem stores them (CPU_PIM.cpp:79-107). */
+typedef struct {
+    int32_t n;
+    const int32_t *F;
+    const int32_t *D;
+} qap_problem;
+
+typedef struct {
+    int32_t R;                   /* replicas per ladder (>= 1)                            */
+    int32_t T;                   /* independent ladders (>= 1)                            */
+    const double *beta_ln;       /* R inverse temperatures in natural-log units, strictly
+                                    positive and ascending (index 0 = hottest); converted
+                                    to exp2 units with the reference's 1.44269504089      */
+    int64_t sweeps;              /* sweeps per replica, a multiple of sweeps_per_exchange */
+    int32_t sweeps_per_exchange; /* NUMITERA (CPU_PIM.cpp:19)                             */
+    uint64_t seed;               /* Philox4x32-10 key                                     */
+    int32_t kick_after;          /* hot/cold kick threshold, 100 in the reference         */
+    int32_t has_goal;            /* stop at the first exchange step with best <= goal     */
+    int64_t goal;
+    double timeout_s;            /* 0 = none; checked between device batches of sweeps    */
+    uint32_t flags;
+    const int32_t *init_perm;    /* optional n: start every replica here (location->facility) */
+} qap_pt_params;
+
+typedef struct {
+    int64_t best_cost;           /* min over every replica's best cost                    */
+    int32_t *best_perm;          /* caller-allocated n (may be NULL)                      */
+    uint64_t attempts;           /* p-dit updates performed = sweeps_done * n * R * T      */
+    uint64_t accepted;           /* updates that moved a facility                          */
+    uint64_t exchanges;          /* ladder swaps by the exchange pass                     */
+    uint64_t kicks;              /* hot/cold kicks                                         */
+    int64_t sweeps_done;         /* per replica                                            */
+    double seconds;              /* wall time of the sweep phase                          */
+} qap_pt_result;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *qap_pt_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int qap_pt_device_count(void);
+
+/* --- one-shot solve: replaces CPU_PIM/GPU_PIM main()'s solve path ------------- */
+int qap_pt_solve_hip(const qap_problem *prob, const qap_pt_params *par, qap_pt_result *res);
+
+/* --- batched solve: many independent instances (uneven n, own ladders) in the same
+ *     device launches; results[k] belongs to problems[k]. ------------------------- */
+int qap_pt_solve_batch_hip(int32_t count, const qap_problem *problems,
+                           const qap_pt_params *params, qap_pt_result *results);
+
+/* --- staged pool (qap::ReplicaPool, qap.cu:28-303: from_file/cu_copy/
+ *     recompute_force_mats/recompute_energies -> create; perform_step +
+ *     percolate_betas -> run; thunk_scores -> result; cu_free -> destroy) ------- */
+typedef struct qap_pt_pool qap_pt_pool;
+
+/* Uploads count instances, builds every initial permutation, force matrix and cost
+ * on the device.  params[k].sweeps is the default length for qap_pt_pool_run. */
+int qap_pt_pool_create(int32_t count, const qap_problem *problems, const qap_pt_params *params,
+                       qap_pt_pool **out);
+/* Advances every unfinished instance by `sweeps` sweeps (a multiple of each
+ * instance's sweeps_per_exchange; <0 = each instance's params.sweeps) and waits.
+ * With kernel_ms != NULL the sweep kernels run without a graph and the summed
+ * device time of every sweep-kernel launch (HIP events on the pool's stream) and
+ * the number of such launches are returned. */
+int qap_pt_pool_run(qap_pt_pool *pool, int64_t sweeps, double *kernel_ms, int64_t *launches);
+int qap_pt_pool_result(qap_pt_pool *pool, int32_t inst, qap_pt_result *res);
+/* Copies instance inst's replica state (each pointer may be NULL). Sizes use
+ * RT = R*T: perm RT*n, E RT, bestE RT (INT64_MAX = no accepted move yet), pos RT
+ * (ladder position, 0 = hottest), accepted RT, best_perms RT*n. */
+int qap_pt_pool_state(qap_pt_pool *pool, int32_t inst, int32_t *perm, int64_t *E, int64_t *bestE,
+                      int32_t *pos, uint64_t *accepted, int32_t *best_perms);
+/* Rebuilds every force matrix and cost from the permutations on the device and
+ * counts disagreeing entries with the incrementally tracked ones (0 = exact). */
+int qap_pt_pool_verify(qap_pt_pool *pool, int64_t *force_mismatches, int64_t *cost_mismatches);
+int qap_pt_pool_destroy(qap_pt_pool *pool);
+
+/* Sweep-kernel shape chosen for an instance of size n ("threads x candidates per
+ * thread, lds|hbm"), for reports. */
+int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

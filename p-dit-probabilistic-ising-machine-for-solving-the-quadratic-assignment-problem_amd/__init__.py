@@ -1,0 +1,302 @@
+"""Python binding of the MI355X parallel-tempering QAP annealer (libqap_pt.so).
+
+The product is the C-ABI in ``include/qap_pt.h`` implemented by HIP kernels for
+gfx950; this module is a thin ``ctypes`` layer over it for tests, benchmarks and
+Python callers.  It mirrors the reference solve path (CPU-PIM/CPU_PIM.cpp:478-685):
+flow/distance matrices, an explicit temperature ladder, sweeps and a seed go in; the
+best permutation and its exact integer cost come out.  There is no CPU fallback:
+loading fails loudly when the HIP library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import instances  # noqa: F401  (re-exported helpers)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libqap_pt.so")
+
+QAP_PT_OK = 0
+QAP_PT_E_INVAL = -1
+QAP_PT_E_NOMEM = -2
+QAP_PT_E_DEVICE = -3
+QAP_PT_E_OVERFLOW = -4
+QAP_PT_E_BUSY = -5
+QAP_PT_INIT_PER_REPLICA = 1
+MAX_N = 1024
+
+# Every symbol the C header declares (checked by the CPU test suite).
+EXPORTS = (
+    "qap_pt_last_error",
+    "qap_pt_device_count",
+    "qap_pt_solve_hip",
+    "qap_pt_solve_batch_hip",
+    "qap_pt_pool_create",
+    "qap_pt_pool_run",
+    "qap_pt_pool_result",
+    "qap_pt_pool_state",
+    "qap_pt_pool_verify",
+    "qap_pt_pool_destroy",
+    "qap_pt_kernel_shape",
+)
+
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+_U64P = C.POINTER(C.c_uint64)
+_F64P = C.POINTER(C.c_double)
+
+
+class QapProblem(C.Structure):
+    _fields_ = [("n", C.c_int32), ("F", _I32P), ("D", _I32P)]
+
+
+class QapPtParams(C.Structure):
+    _fields_ = [
+        ("R", C.c_int32),
+        ("T", C.c_int32),
+        ("beta_ln", _F64P),
+        ("sweeps", C.c_int64),
+        ("sweeps_per_exchange", C.c_int32),
+        ("seed", C.c_uint64),
+        ("kick_after", C.c_int32),
+        ("has_goal", C.c_int32),
+        ("goal", C.c_int64),
+        ("timeout_s", C.c_double),
+        ("flags", C.c_uint32),
+        ("init_perm", _I32P),
+    ]
+
+
+class QapPtResult(C.Structure):
+    _fields_ = [
+        ("best_cost", C.c_int64),
+        ("best_perm", _I32P),
+        ("attempts", C.c_uint64),
+        ("accepted", C.c_uint64),
+        ("exchanges", C.c_uint64),
+        ("kicks", C.c_uint64),
+        ("sweeps_done", C.c_int64),
+        ("seconds", C.c_double),
+    ]
+
+
+class QapError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"qap_pt error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libqap_pt.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    lib.qap_pt_last_error.restype = C.c_char_p
+    lib.qap_pt_device_count.restype = C.c_int
+    lib.qap_pt_solve_hip.argtypes = [C.POINTER(QapProblem), C.POINTER(QapPtParams),
+                                     C.POINTER(QapPtResult)]
+    lib.qap_pt_solve_batch_hip.argtypes = [C.c_int32, C.POINTER(QapProblem),
+                                           C.POINTER(QapPtParams), C.POINTER(QapPtResult)]
+    lib.qap_pt_pool_create.argtypes = [C.c_int32, C.POINTER(QapProblem),
+                                       C.POINTER(QapPtParams), C.POINTER(C.c_void_p)]
+    lib.qap_pt_pool_run.argtypes = [C.c_void_p, C.c_int64, _F64P, _I64P]
+    lib.qap_pt_pool_result.argtypes = [C.c_void_p, C.c_int32, C.POINTER(QapPtResult)]
+    lib.qap_pt_pool_state.argtypes = [C.c_void_p, C.c_int32, _I32P, _I64P, _I64P, _I32P,
+                                      _U64P, _I32P]
+    lib.qap_pt_pool_verify.argtypes = [C.c_void_p, _I64P, _I64P]
+    lib.qap_pt_pool_destroy.argtypes = [C.c_void_p]
+    lib.qap_pt_kernel_shape.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != QAP_PT_OK:
+        raise QapError(rc, load_library().qap_pt_last_error().decode())
+
+
+def device_count() -> int:
+    return int(load_library().qap_pt_device_count())
+
+
+def kernel_shape(n: int) -> str:
+    buf = C.create_string_buffer(64)
+    _check(load_library().qap_pt_kernel_shape(n, buf, 64))
+    return buf.value.decode()
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+@dataclass
+class Instance:
+    """One QAP instance plus its annealing parameters (the reference's flags)."""
+    F: np.ndarray
+    D: np.ndarray
+    beta_ln: np.ndarray
+    sweeps: int
+    R: int = 0
+    T: int = 1
+    sweeps_per_exchange: int = 1
+    seed: int = 0
+    kick_after: int = 100
+    goal: int | None = None
+    timeout_s: float = 0.0
+    per_replica_init: bool = False
+    init_perm: np.ndarray | None = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    def __post_init__(self):
+        self.F = np.ascontiguousarray(self.F, dtype=np.int32)
+        self.D = np.ascontiguousarray(self.D, dtype=np.int32)
+        self.beta_ln = np.ascontiguousarray(self.beta_ln, dtype=np.float64)
+        if self.R == 0:
+            self.R = int(self.beta_ln.size)
+        if self.init_perm is not None:
+            self.init_perm = np.ascontiguousarray(self.init_perm, dtype=np.int32)
+
+    @property
+    def n(self) -> int:
+        return int(self.F.shape[0])
+
+    def c_structs(self):
+        prob = QapProblem(self.n, _ptr(self.F, C.c_int32), _ptr(self.D, C.c_int32))
+        par = QapPtParams()
+        par.R, par.T = self.R, self.T
+        par.beta_ln = _ptr(self.beta_ln, C.c_double)
+        par.sweeps = int(self.sweeps)
+        par.sweeps_per_exchange = int(self.sweeps_per_exchange)
+        par.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
+        par.kick_after = int(self.kick_after)
+        par.has_goal = 0 if self.goal is None else 1
+        par.goal = 0 if self.goal is None else int(self.goal)
+        par.timeout_s = float(self.timeout_s)
+        par.flags = QAP_PT_INIT_PER_REPLICA if self.per_replica_init else 0
+        par.init_perm = (_ptr(self.init_perm, C.c_int32) if self.init_perm is not None
+                         else C.cast(None, _I32P))
+        return prob, par
+
+
+def _result_dict(res: QapPtResult, perm: np.ndarray) -> dict:
+    return {
+        "best_cost": int(res.best_cost),
+        "best_perm": perm,
+        "attempts": int(res.attempts),
+        "accepted": int(res.accepted),
+        "exchanges": int(res.exchanges),
+        "kicks": int(res.kicks),
+        "sweeps_done": int(res.sweeps_done),
+        "seconds": float(res.seconds),
+    }
+
+
+def solve(inst: Instance) -> dict:
+    """qap_pt_solve_hip: one instance, run to inst.sweeps."""
+    lib = load_library()
+    prob, par = inst.c_structs()
+    perm = np.zeros(inst.n, dtype=np.int32)
+    res = QapPtResult()
+    res.best_perm = _ptr(perm, C.c_int32)
+    _check(lib.qap_pt_solve_hip(C.byref(prob), C.byref(par), C.byref(res)))
+    return _result_dict(res, perm)
+
+
+def solve_batch(insts: list[Instance]) -> list[dict]:
+    """qap_pt_solve_batch_hip: every instance in the same device launches."""
+    lib = load_library()
+    k = len(insts)
+    probs = (QapProblem * k)()
+    pars = (QapPtParams * k)()
+    ress = (QapPtResult * k)()
+    perms = []
+    for x, inst in enumerate(insts):
+        probs[x], pars[x] = inst.c_structs()
+        p = np.zeros(inst.n, dtype=np.int32)
+        perms.append(p)
+        ress[x].best_perm = _ptr(p, C.c_int32)
+    _check(lib.qap_pt_solve_batch_hip(k, probs, pars, ress))
+    return [_result_dict(ress[x], perms[x]) for x in range(k)]
+
+
+class Pool:
+    """Staged pool (the reference's qap::ReplicaPool, qap.cu:28-303)."""
+
+    def __init__(self, insts: list[Instance]):
+        self.lib = load_library()
+        self.insts = list(insts)
+        k = len(self.insts)
+        self._probs = (QapProblem * k)()
+        self._pars = (QapPtParams * k)()
+        for x, inst in enumerate(self.insts):
+            self._probs[x], self._pars[x] = inst.c_structs()
+        self.handle = C.c_void_p()
+        _check(self.lib.qap_pt_pool_create(k, self._probs, self._pars, C.byref(self.handle)))
+
+    def run(self, sweeps: int = -1, timed: bool = False):
+        """Advance every instance; with timed=True returns (kernel_ms, launches)."""
+        if timed:
+            ms = C.c_double(0.0)
+            nl = C.c_int64(0)
+            _check(self.lib.qap_pt_pool_run(self.handle, int(sweeps), C.byref(ms), C.byref(nl)))
+            return float(ms.value), int(nl.value)
+        _check(self.lib.qap_pt_pool_run(self.handle, int(sweeps), None, None))
+        return None
+
+    def result(self, k: int = 0) -> dict:
+        perm = np.zeros(self.insts[k].n, dtype=np.int32)
+        res = QapPtResult()
+        res.best_perm = _ptr(perm, C.c_int32)
+        _check(self.lib.qap_pt_pool_result(self.handle, k, C.byref(res)))
+        return _result_dict(res, perm)
+
+    def state(self, k: int = 0) -> dict:
+        inst = self.insts[k]
+        rt, n = inst.R * inst.T, inst.n
+        perm = np.zeros((rt, n), dtype=np.int32)
+        bperm = np.zeros((rt, n), dtype=np.int32)
+        E = np.zeros(rt, dtype=np.int64)
+        bE = np.zeros(rt, dtype=np.int64)
+        pos = np.zeros(rt, dtype=np.int32)
+        acc = np.zeros(rt, dtype=np.uint64)
+        _check(self.lib.qap_pt_pool_state(self.handle, k, _ptr(perm, C.c_int32),
+                                          _ptr(E, C.c_int64), _ptr(bE, C.c_int64),
+                                          _ptr(pos, C.c_int32), _ptr(acc, C.c_uint64),
+                                          _ptr(bperm, C.c_int32)))
+        return {"perm": perm, "E": E, "bestE": bE, "pos": pos, "accepted": acc,
+                "best_perms": bperm}
+
+    def verify(self) -> tuple[int, int]:
+        fm = C.c_int64(0)
+        cm = C.c_int64(0)
+        _check(self.lib.qap_pt_pool_verify(self.handle, C.byref(fm), C.byref(cm)))
+        return int(fm.value), int(cm.value)
+
+    def close(self):
+        if self.handle:
+            self.lib.qap_pt_pool_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

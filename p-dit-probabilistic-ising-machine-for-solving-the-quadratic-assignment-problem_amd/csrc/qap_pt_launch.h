@@ -1,0 +1,36 @@
+// qap_pt_launch.h -- host-callable launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qap_pt_device.h"
+
+namespace qap {
+
+// LDS available to one workgroup on gfx950 (160 KiB per CU).
+constexpr int kMaxLdsBytes = 160 * 1024;
+
+struct SweepShape {
+    int nt;      // threads per workgroup (multiple of 64)
+    int cpl;     // candidates per thread: n <= nt * cpl
+    bool lds_i;  // force matrix staged in LDS for the launch
+    bool sym;    // symmetric, zero-diagonal F and D
+};
+
+bool sweep_shape_supported(int nt, int cpl);
+int sweep_lds_bytes(int n, int ld, bool lds_i);
+
+hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,
+                        int lds_bytes, hipStream_t s);
+hipError_t launch_exchange(const PoolDev &P, int count, hipStream_t s);
+hipError_t launch_set_limits(InstState *state, const int64_t *limits, int count, hipStream_t s);
+hipError_t launch_halt(InstState *state, int count, hipStream_t s);
+hipError_t launch_force(const int32_t *F, const int32_t *D, const int32_t *perm, int n, int ld,
+                        int nrep, int32_t *Iout, hipStream_t s);
+hipError_t launch_energy(const int32_t *F, const int32_t *D, const int32_t *perm, int n, int nrep,
+                         int64_t *Eout, hipStream_t s);
+hipError_t launch_count_mismatch(const int32_t *a, const int32_t *b, int n, int ld,
+                                 int64_t total_rows, unsigned long long *out, hipStream_t s);
+
+}  // namespace qap

@@ -1,0 +1,791 @@
+// qap_pt_pool.cpp -- host side of the C-ABI (include/qap_pt.h): validation, device
+// layout, set-up, hipGraph-captured sweep+exchange steps, results.
+//
+// Device layout (all instances of a pool concatenated, structure of arrays):
+//   F, D, Ft, Dt      int32 n*n per instance (transposes feed the asymmetric path)
+//   Fdiag, Ddiag      int32 n per instance
+//   perm, pinv,
+//   best_perm         int32 n per replica      (replica = instance-major, ladder, rung)
+//   I                 int32 n*ld per replica   (ld = n rounded up to 4: 16-byte rows)
+//   E, bestE          int64 per replica; accepted uint64; pos int32 (ladder position)
+//   order             int32 per replica slot (position -> replica, per ladder)
+//   tss, ladder_best  per ladder
+//   beta2f, c20       per ladder position (float exp2-units beta, double 20/beta)
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/qap_pt.h"
+#include "qap_pt_launch.h"
+
+using namespace qap;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(QAP_PT_E_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                   \
+    } while (0)
+
+// Reference ln -> exp2 conversion constant (CPU_PIM.cpp:528, GPU_PIM.cu:792-793).
+constexpr double kLog2e = 1.44269504089;
+
+struct Bounds {
+    int64_t force, delta, maxD, maxF;
+};
+
+Bounds compute_bounds(int n, const int32_t *F, const int32_t *D) {
+    // |I[i][a]| <= min(maxF * max_i sum_j |D_ij|+|D_ji|, maxD * max_a sum_c |F_ac|+|F_ca|)
+    int64_t maxD = 0, maxF = 0, sD = 0, sF = 0;
+    for (int64_t x = 0; x < (int64_t)n * n; ++x) {
+        maxD = std::max<int64_t>(maxD, std::llabs((long long)D[x]));
+        maxF = std::max<int64_t>(maxF, std::llabs((long long)F[x]));
+    }
+    for (int i = 0; i < n; ++i) {
+        int64_t rd = 0, rf = 0;
+        for (int j = 0; j < n; ++j) {
+            rd += std::llabs((long long)D[(int64_t)i * n + j]) + std::llabs((long long)D[(int64_t)j * n + i]);
+            rf += std::llabs((long long)F[(int64_t)i * n + j]) + std::llabs((long long)F[(int64_t)j * n + i]);
+        }
+        sD = std::max(sD, rd);
+        sF = std::max(sF, rf);
+    }
+    Bounds b;
+    b.force = std::min(maxF * sD, maxD * sF);
+    b.delta = 4 * b.force + 16 * maxD * maxF;
+    b.maxD = maxD;
+    b.maxF = maxF;
+    return b;
+}
+
+bool is_sym_zero_diag(int n, const int32_t *M) {
+    for (int i = 0; i < n; ++i) {
+        if (M[(int64_t)i * n + i] != 0) return false;
+        for (int j = i + 1; j < n; ++j)
+            if (M[(int64_t)i * n + j] != M[(int64_t)j * n + i]) return false;
+    }
+    return true;
+}
+
+int round4i(int x) { return (x + 3) & ~3; }
+
+SweepShape choose_shape(int n, bool sym) {
+    SweepShape s{};
+    s.sym = sym;
+    if (n <= 64) {
+        s.nt = 64;
+        s.cpl = 1;
+    } else if (n <= 128) {
+        s.nt = 64;
+        s.cpl = 2;
+    } else if (n <= 256) {
+        s.nt = 256;
+        s.cpl = 1;
+    } else if (n <= 512) {
+        s.nt = 256;
+        s.cpl = 2;
+    } else {
+        s.nt = 256;
+        s.cpl = 4;
+    }
+    // Optional override for tuning sweeps: QAP_PT_SHAPE="nt,cpl" (must cover n).
+    if (const char *env = std::getenv("QAP_PT_SHAPE")) {
+        int nt = 0, cpl = 0;
+        if (std::sscanf(env, "%d,%d", &nt, &cpl) == 2 && sweep_shape_supported(nt, cpl) &&
+            nt * cpl >= n) {
+            s.nt = nt;
+            s.cpl = cpl;
+        }
+    }
+    const int ld = round4i(n);
+    s.lds_i = sweep_lds_bytes(n, ld, true) <= kMaxLdsBytes;
+    if (const char *env = std::getenv("QAP_PT_FORCE_HBM"))
+        if (env[0] == '1') s.lds_i = false;
+    return s;
+}
+
+// Initial permutation (CPU_PIM.cpp:134-161): identity, then for i = 0..n-1 swap
+// positions i and j, j uniform in [0,n) from Philox ctr = {i, stream, 0, tag}.
+void init_perm(int n, uint64_t seed, uint32_t stream, uint32_t tag, int32_t *perm) {
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int i = 0; i < n; ++i) {
+        const u32x4 x = philox4x32_10(u32x4{(uint32_t)i, stream, 0u, tag}, (uint32_t)seed,
+                                      (uint32_t)(seed >> 32));
+        const int j = (int)(((uint64_t)x.x * (uint64_t)n) >> 32);
+        if (i != j) std::swap(perm[i], perm[j]);
+    }
+}
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void **)p, count * sizeof(T));
+}
+
+}  // namespace
+
+struct Bucket {
+    SweepShape shape;
+    int lds_bytes = 0;
+    std::vector<int32_t> reps;
+    int32_t *d_list = nullptr;
+};
+
+struct qap_pt_pool {
+    int device = 0;
+    int count = 0;
+    hipStream_t stream = nullptr;
+    std::vector<qap_pt_params> params;
+    std::vector<InstDesc> desc;
+    std::vector<SweepShape> shapes;
+    std::vector<Bucket> buckets;
+    std::vector<int64_t> limits;  // mega-iteration limit per instance (host mirror)
+    int total_reps = 0, total_ladders = 0;
+    int64_t total_p = 0, total_I = 0, total_mat = 0, total_vec = 0, total_beta = 0;
+    PoolDev P{};
+    // owned device buffers
+    InstDesc *d_inst = nullptr;
+    InstState *d_state = nullptr;
+    int32_t *d_rep_inst = nullptr, *d_F = nullptr, *d_D = nullptr, *d_Ft = nullptr, *d_Dt = nullptr;
+    int32_t *d_Fdiag = nullptr, *d_Ddiag = nullptr;
+    float *d_beta2f = nullptr;
+    double *d_c20 = nullptr;
+    int32_t *d_perm = nullptr, *d_pinv = nullptr, *d_best_perm = nullptr, *d_I = nullptr;
+    int64_t *d_E = nullptr, *d_bestE = nullptr, *d_limits = nullptr;
+    uint64_t *d_accepted = nullptr;
+    int32_t *d_pos = nullptr, *d_order = nullptr, *d_tss = nullptr;
+    int64_t *d_ladder_best = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_depth = 0;
+    double seconds = 0.0;
+};
+
+namespace {
+
+int validate(const qap_problem &pb, const qap_pt_params &pr, int k) {
+    if (pb.n < 1 || pb.n > QAP_PT_MAX_N)
+        return fail(QAP_PT_E_INVAL, "instance %d: n=%d outside [1,%d]", k, pb.n, QAP_PT_MAX_N);
+    if (!pb.F || !pb.D) return fail(QAP_PT_E_INVAL, "instance %d: F or D is NULL", k);
+    if (pr.R < 1 || pr.T < 1) return fail(QAP_PT_E_INVAL, "instance %d: R and T must be >= 1", k);
+    if ((int64_t)pr.R * pr.T > (1 << 24))
+        return fail(QAP_PT_E_INVAL, "instance %d: R*T too large", k);
+    if (!pr.beta_ln) return fail(QAP_PT_E_INVAL, "instance %d: beta_ln is NULL", k);
+    for (int l = 0; l < pr.R; ++l) {
+        if (!(pr.beta_ln[l] > 0.0) || !(pr.beta_ln[l] < 1e30))
+            return fail(QAP_PT_E_INVAL, "instance %d: beta_ln[%d] must be finite and > 0", k, l);
+        if (l && pr.beta_ln[l] < pr.beta_ln[l - 1])
+            return fail(QAP_PT_E_INVAL, "instance %d: beta_ln must be ascending", k);
+    }
+    if (pr.sweeps_per_exchange < 1)
+        return fail(QAP_PT_E_INVAL, "instance %d: sweeps_per_exchange must be >= 1", k);
+    if (pr.sweeps < 0 || pr.sweeps % pr.sweeps_per_exchange)
+        return fail(QAP_PT_E_INVAL, "instance %d: sweeps must be >= 0 and a multiple of "
+                                    "sweeps_per_exchange", k);
+    if (pr.sweeps > (int64_t)0xffffffffLL)
+        return fail(QAP_PT_E_INVAL, "instance %d: sweeps must be < 2^32", k);
+    if (pr.kick_after < 0) return fail(QAP_PT_E_INVAL, "instance %d: kick_after < 0", k);
+    if (pr.timeout_s < 0) return fail(QAP_PT_E_INVAL, "instance %d: timeout_s < 0", k);
+    if (pr.init_perm) {
+        std::vector<char> seen(pb.n, 0);
+        for (int i = 0; i < pb.n; ++i) {
+            const int v = pr.init_perm[i];
+            if (v < 0 || v >= pb.n || seen[v])
+                return fail(QAP_PT_E_INVAL, "instance %d: init_perm is not a permutation", k);
+            seen[v] = 1;
+        }
+    }
+    const Bounds b = compute_bounds(pb.n, pb.F, pb.D);
+    if (b.delta >= ((int64_t)1 << 30) || b.force + 8 * b.maxD * b.maxF >= ((int64_t)1 << 31))
+        return fail(QAP_PT_E_OVERFLOW,
+                    "instance %d: |force| bound %lld / |delta| bound %lld exceed the int32 range "
+                    "the sweep kernels prove exact",
+                    k, (long long)b.force, (long long)b.delta);
+    return QAP_PT_OK;
+}
+
+void pool_free(qap_pt_pool *p) {
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (auto &b : p->buckets)
+        if (b.d_list) (void)hipFree(b.d_list);
+    void *bufs[] = {p->d_inst,  p->d_state,     p->d_rep_inst, p->d_F,      p->d_D,
+                    p->d_Ft,    p->d_Dt,        p->d_Fdiag,    p->d_Ddiag,  p->d_beta2f,
+                    p->d_c20,   p->d_perm,      p->d_pinv,     p->d_best_perm, p->d_I,
+                    p->d_E,     p->d_bestE,     p->d_limits,   p->d_accepted, p->d_pos,
+                    p->d_order, p->d_tss,       p->d_ladder_best};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+int build_graph(qap_pt_pool *p, int depth) {
+    if (p->graph_exec) {
+        (void)hipGraphExecDestroy(p->graph_exec);
+        p->graph_exec = nullptr;
+    }
+    if (p->graph) {
+        (void)hipGraphDestroy(p->graph);
+        p->graph = nullptr;
+    }
+    HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    for (int m = 0; m < depth; ++m) {
+        hipError_t e = launch_exchange(p->P, p->count, p->stream);
+        for (auto &b : p->buckets) {
+            if (e != hipSuccess) break;
+            e = launch_sweep(p->P, b.shape, b.d_list, (int)b.reps.size(), b.lds_bytes, p->stream);
+        }
+        if (e != hipSuccess) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(p->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            return fail(QAP_PT_E_DEVICE, "kernel launch during capture failed: %s",
+                        hipGetErrorString(e));
+        }
+    }
+    HIP_TRY(hipStreamEndCapture(p->stream, &p->graph));
+    HIP_TRY(hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0));
+    p->graph_depth = depth;
+    return QAP_PT_OK;
+}
+
+int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_params *params,
+                     qap_pt_pool **out) {
+    if (!out) return fail(QAP_PT_E_INVAL, "out is NULL");
+    *out = nullptr;
+    if (count < 1 || !problems || !params) return fail(QAP_PT_E_INVAL, "empty instance list");
+    for (int k = 0; k < count; ++k) {
+        const int rc = validate(problems[k], params[k], k);
+        if (rc) return rc;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(QAP_PT_E_DEVICE, "no HIP device visible");
+
+    qap_pt_pool *p = new qap_pt_pool();
+    (void)hipGetDevice(&p->device);
+    p->count = count;
+    p->params.assign(params, params + count);
+    p->desc.resize(count);
+    p->shapes.resize(count);
+    p->limits.assign(count, 0);
+
+    std::vector<int32_t> hF, hD, hFt, hDt, hFd, hDd, rep_inst, perm, pinv, pos, order;
+    std::vector<float> beta2f;
+    std::vector<double> c20;
+    std::map<std::tuple<int, int, int, int>, int> bucket_of;
+    for (int k = 0; k < count; ++k) {
+        const qap_problem &pb = problems[k];
+        const qap_pt_params &pr = params[k];
+        const int n = pb.n;
+        InstDesc &d = p->desc[k];
+        const bool sym = is_sym_zero_diag(n, pb.F) && is_sym_zero_diag(n, pb.D);
+        const SweepShape sh = choose_shape(n, sym);
+        p->shapes[k] = sh;
+        d.n = n;
+        d.ld = round4i(n);
+        d.R = pr.R;
+        d.T = pr.T;
+        d.spe = pr.sweeps_per_exchange;
+        d.kick_after = pr.kick_after;
+        d.has_goal = pr.has_goal ? 1 : 0;
+        d.sym = sym ? 1 : 0;
+        d.goal = pr.goal;
+        d.mat_off = p->total_mat;
+        d.vec_off = p->total_vec;
+        d.p_base = p->total_p;
+        d.I_base = p->total_I;
+        d.rep_base = p->total_reps;
+        d.ladder_base = p->total_ladders;
+        d.beta_off = (int32_t)p->total_beta;
+        d.seed_lo = (uint32_t)pr.seed;
+        d.seed_hi = (uint32_t)(pr.seed >> 32);
+        const int RT = pr.R * pr.T;
+        p->total_mat += (int64_t)n * n;
+        p->total_vec += n;
+        p->total_p += (int64_t)RT * n;
+        p->total_I += (int64_t)RT * n * d.ld;
+        p->total_reps += RT;
+        p->total_ladders += pr.T;
+        p->total_beta += pr.R;
+
+        hF.insert(hF.end(), pb.F, pb.F + (int64_t)n * n);
+        hD.insert(hD.end(), pb.D, pb.D + (int64_t)n * n);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                hFt.push_back(pb.F[(int64_t)j * n + i]);
+                hDt.push_back(pb.D[(int64_t)j * n + i]);
+            }
+        for (int i = 0; i < n; ++i) {
+            hFd.push_back(pb.F[(int64_t)i * n + i]);
+            hDd.push_back(pb.D[(int64_t)i * n + i]);
+        }
+        for (int l = 0; l < pr.R; ++l) {
+            const double b2 = pr.beta_ln[l] * kLog2e;
+            beta2f.push_back((float)b2);
+            c20.push_back(20.0 / b2);
+        }
+        std::vector<int32_t> pi(n);
+        for (int t = 0; t < pr.T; ++t) {
+            if (!(pr.flags & QAP_PT_INIT_PER_REPLICA)) {
+                if (pr.init_perm) std::copy(pr.init_perm, pr.init_perm + n, pi.begin());
+                else init_perm(n, pr.seed, (uint32_t)t, 1u, pi.data());
+            }
+            for (int l = 0; l < pr.R; ++l) {
+                if (pr.flags & QAP_PT_INIT_PER_REPLICA) {
+                    if (pr.init_perm) std::copy(pr.init_perm, pr.init_perm + n, pi.begin());
+                    else init_perm(n, pr.seed, (uint32_t)(t * pr.R + l), 2u, pi.data());
+                }
+                perm.insert(perm.end(), pi.begin(), pi.end());
+                std::vector<int32_t> inv(n);
+                for (int i = 0; i < n; ++i) inv[pi[i]] = i;
+                pinv.insert(pinv.end(), inv.begin(), inv.end());
+                pos.push_back(l);
+                order.push_back(l);
+                rep_inst.push_back(k);
+            }
+        }
+        const auto key = std::make_tuple(sh.nt, sh.cpl, (int)sh.lds_i, (int)sh.sym);
+        auto it = bucket_of.find(key);
+        int bi;
+        if (it == bucket_of.end()) {
+            bi = (int)p->buckets.size();
+            bucket_of[key] = bi;
+            p->buckets.emplace_back();
+            p->buckets[bi].shape = sh;
+        } else {
+            bi = it->second;
+        }
+        Bucket &B = p->buckets[bi];
+        B.lds_bytes = std::max(B.lds_bytes, sweep_lds_bytes(n, d.ld, sh.lds_i));
+        for (int r = 0; r < RT; ++r) B.reps.push_back(d.rep_base + r);
+    }
+
+    auto bail = [&](int rc) {
+        pool_free(p);
+        return rc;
+    };
+#define POOL_TRY(expr)                                                                           \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return bail(fail(e_ == hipErrorOutOfMemory ? QAP_PT_E_NOMEM : QAP_PT_E_DEVICE,       \
+                             "%s failed: %s", #expr, hipGetErrorString(e_)));                    \
+    } while (0)
+
+    POOL_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    const int RTall = p->total_reps;
+    POOL_TRY(dalloc(&p->d_inst, count));
+    POOL_TRY(dalloc(&p->d_state, count));
+    POOL_TRY(dalloc(&p->d_limits, count));
+    POOL_TRY(dalloc(&p->d_rep_inst, RTall));
+    POOL_TRY(dalloc(&p->d_F, p->total_mat));
+    POOL_TRY(dalloc(&p->d_D, p->total_mat));
+    POOL_TRY(dalloc(&p->d_Ft, p->total_mat));
+    POOL_TRY(dalloc(&p->d_Dt, p->total_mat));
+    POOL_TRY(dalloc(&p->d_Fdiag, p->total_vec));
+    POOL_TRY(dalloc(&p->d_Ddiag, p->total_vec));
+    POOL_TRY(dalloc(&p->d_beta2f, p->total_beta));
+    POOL_TRY(dalloc(&p->d_c20, p->total_beta));
+    POOL_TRY(dalloc(&p->d_perm, p->total_p));
+    POOL_TRY(dalloc(&p->d_pinv, p->total_p));
+    POOL_TRY(dalloc(&p->d_best_perm, p->total_p));
+    POOL_TRY(dalloc(&p->d_I, p->total_I));
+    POOL_TRY(dalloc(&p->d_E, RTall));
+    POOL_TRY(dalloc(&p->d_bestE, RTall));
+    POOL_TRY(dalloc(&p->d_accepted, RTall));
+    POOL_TRY(dalloc(&p->d_pos, RTall));
+    POOL_TRY(dalloc(&p->d_order, RTall));
+    POOL_TRY(dalloc(&p->d_tss, p->total_ladders));
+    POOL_TRY(dalloc(&p->d_ladder_best, p->total_ladders));
+    for (auto &b : p->buckets) {
+        POOL_TRY(dalloc(&b.d_list, b.reps.size()));
+        POOL_TRY(hipMemcpy(b.d_list, b.reps.data(), b.reps.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice));
+    }
+
+    std::vector<InstState> st(count);
+    for (auto &s : st) {
+        std::memset(&s, 0, sizeof(s));
+        s.global_best = INT64_MAX;
+    }
+    std::vector<int64_t> bestE(RTall, INT64_MAX), ladder_best(p->total_ladders, INT64_MAX);
+    std::vector<uint64_t> acc(RTall, 0);
+    std::vector<int32_t> tss(p->total_ladders, 0);
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    POOL_TRY(up(p->d_inst, p->desc.data(), sizeof(InstDesc) * count));
+    POOL_TRY(up(p->d_state, st.data(), sizeof(InstState) * count));
+    POOL_TRY(up(p->d_rep_inst, rep_inst.data(), 4 * rep_inst.size()));
+    POOL_TRY(up(p->d_F, hF.data(), 4 * hF.size()));
+    POOL_TRY(up(p->d_D, hD.data(), 4 * hD.size()));
+    POOL_TRY(up(p->d_Ft, hFt.data(), 4 * hFt.size()));
+    POOL_TRY(up(p->d_Dt, hDt.data(), 4 * hDt.size()));
+    POOL_TRY(up(p->d_Fdiag, hFd.data(), 4 * hFd.size()));
+    POOL_TRY(up(p->d_Ddiag, hDd.data(), 4 * hDd.size()));
+    POOL_TRY(up(p->d_beta2f, beta2f.data(), 4 * beta2f.size()));
+    POOL_TRY(up(p->d_c20, c20.data(), 8 * c20.size()));
+    POOL_TRY(up(p->d_perm, perm.data(), 4 * perm.size()));
+    POOL_TRY(up(p->d_pinv, pinv.data(), 4 * pinv.size()));
+    POOL_TRY(up(p->d_best_perm, perm.data(), 4 * perm.size()));
+    POOL_TRY(up(p->d_bestE, bestE.data(), 8 * bestE.size()));
+    POOL_TRY(up(p->d_accepted, acc.data(), 8 * acc.size()));
+    POOL_TRY(up(p->d_pos, pos.data(), 4 * pos.size()));
+    POOL_TRY(up(p->d_order, order.data(), 4 * order.size()));
+    POOL_TRY(up(p->d_tss, tss.data(), 4 * tss.size()));
+    POOL_TRY(up(p->d_ladder_best, ladder_best.data(), 8 * ladder_best.size()));
+    POOL_TRY(hipMemset(p->d_I, 0, sizeof(int32_t) * std::max<int64_t>(p->total_I, 1)));
+
+    PoolDev &P = p->P;
+    P.inst = p->d_inst;
+    P.state = p->d_state;
+    P.rep_inst = p->d_rep_inst;
+    P.F = p->d_F;
+    P.D = p->d_D;
+    P.Ft = p->d_Ft;
+    P.Dt = p->d_Dt;
+    P.Fdiag = p->d_Fdiag;
+    P.Ddiag = p->d_Ddiag;
+    P.beta2f = p->d_beta2f;
+    P.c20 = p->d_c20;
+    P.perm = p->d_perm;
+    P.pinv = p->d_pinv;
+    P.best_perm = p->d_best_perm;
+    P.I = p->d_I;
+    P.E = p->d_E;
+    P.bestE = p->d_bestE;
+    P.accepted = p->d_accepted;
+    P.pos = p->d_pos;
+    P.order = p->d_order;
+    P.tss = p->d_tss;
+    P.ladder_best = p->d_ladder_best;
+
+    // Force matrices (CPU_PIM.cpp:164-181) and costs (:184-192) on the device.
+    for (int k = 0; k < count; ++k) {
+        const InstDesc &d = p->desc[k];
+        const int RT = d.R * d.T;
+        POOL_TRY(launch_force(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n,
+                              d.ld, RT, p->d_I + d.I_base, p->stream));
+        POOL_TRY(launch_energy(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n,
+                               RT, p->d_E + d.rep_base, p->stream));
+    }
+    POOL_TRY(hipStreamSynchronize(p->stream));
+#undef POOL_TRY
+    *out = p;
+    return QAP_PT_OK;
+}
+
+int pool_run_impl(qap_pt_pool *p, int64_t sweeps, double *kernel_ms, int64_t *launches) {
+    if (!p) return fail(QAP_PT_E_INVAL, "pool is NULL");
+    HIP_TRY(hipSetDevice(p->device));
+    int64_t max_new = 0;
+    bool any_goal = false;
+    double timeout = 0.0;
+    for (int k = 0; k < p->count; ++k) {
+        const qap_pt_params &pr = p->params[k];
+        int64_t megas;
+        if (sweeps < 0) {
+            megas = pr.sweeps / pr.sweeps_per_exchange - p->limits[k];
+            if (megas < 0) megas = 0;
+        } else {
+            if (sweeps % pr.sweeps_per_exchange)
+                return fail(QAP_PT_E_INVAL, "instance %d: sweeps %lld not a multiple of %d", k,
+                            (long long)sweeps, pr.sweeps_per_exchange);
+            megas = sweeps / pr.sweeps_per_exchange;
+        }
+        if ((p->limits[k] + megas) * pr.sweeps_per_exchange > (int64_t)0xffffffffLL)
+            return fail(QAP_PT_E_INVAL, "instance %d: total sweeps would exceed 2^32", k);
+        p->limits[k] += megas;
+        max_new = std::max(max_new, megas);
+        any_goal = any_goal || pr.has_goal;
+        if (pr.timeout_s > 0 && (timeout == 0.0 || pr.timeout_s < timeout)) timeout = pr.timeout_s;
+    }
+    HIP_TRY(hipMemcpyAsync(p->d_limits, p->limits.data(), sizeof(int64_t) * p->count,
+                           hipMemcpyHostToDevice, p->stream));
+    HIP_TRY(launch_set_limits(p->d_state, p->d_limits, p->count, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+
+    const auto t0 = std::chrono::steady_clock::now();
+    auto elapsed = [&]() {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    if (kernel_ms) {
+        // Timed mode: every sweep bucket launch bracketed by events on the pool stream.
+        const int chunk = 256;
+        std::vector<hipEvent_t> ev(2 * chunk * std::max<size_t>(p->buckets.size(), 1));
+        for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+        double total = 0.0;
+        int64_t nl = 0;
+        int64_t m = 0;
+        int rc = QAP_PT_OK;
+        while (m < max_new && rc == QAP_PT_OK) {
+            int used = 0;
+            for (int c = 0; c < chunk && m < max_new; ++c, ++m) {
+                if (launch_exchange(p->P, p->count, p->stream) != hipSuccess) {
+                    rc = fail(QAP_PT_E_DEVICE, "exchange launch failed");
+                    break;
+                }
+                for (auto &b : p->buckets) {
+                    (void)hipEventRecord(ev[2 * used], p->stream);
+                    if (launch_sweep(p->P, b.shape, b.d_list, (int)b.reps.size(), b.lds_bytes,
+                                     p->stream) != hipSuccess) {
+                        rc = fail(QAP_PT_E_DEVICE, "sweep launch failed");
+                        break;
+                    }
+                    (void)hipEventRecord(ev[2 * used + 1], p->stream);
+                    ++used;
+                }
+                if (rc) break;
+            }
+            if (hipStreamSynchronize(p->stream) != hipSuccess) {
+                rc = fail(QAP_PT_E_DEVICE, "stream synchronize failed");
+                break;
+            }
+            for (int u = 0; u < used; ++u) {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, ev[2 * u], ev[2 * u + 1]);
+                total += ms;
+            }
+            nl += used;
+            if (timeout > 0 && elapsed() > timeout) {
+                (void)launch_halt(p->d_state, p->count, p->stream);
+                break;
+            }
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        if (rc) return rc;
+        *kernel_ms = total;
+        if (launches) *launches = nl;
+    } else {
+        const int depth = (int)std::max<int64_t>(1, std::min<int64_t>(max_new, 32));
+        if (max_new > 0 && (!p->graph_exec || p->graph_depth != depth)) {
+            const int rc = build_graph(p, depth);
+            if (rc) return rc;
+        }
+        for (int64_t m = 0; m < max_new; m += p->graph_depth) {
+            HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
+            if (any_goal || timeout > 0) {
+                HIP_TRY(hipStreamSynchronize(p->stream));
+                if (timeout > 0 && elapsed() > timeout) {
+                    HIP_TRY(launch_halt(p->d_state, p->count, p->stream));
+                    break;
+                }
+                if (any_goal) {
+                    std::vector<InstState> st(p->count);
+                    HIP_TRY(hipMemcpy(st.data(), p->d_state, sizeof(InstState) * p->count,
+                                      hipMemcpyDeviceToHost));
+                    bool all_done = true;
+                    for (int k = 0; k < p->count; ++k)
+                        all_done = all_done && (st[k].halted || st[k].megas_done >= p->limits[k]);
+                    if (all_done) break;
+                }
+            }
+        }
+        if (launches) *launches = 0;
+    }
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    p->seconds += elapsed();
+    return QAP_PT_OK;
+}
+
+int pool_result_impl(qap_pt_pool *p, int inst, qap_pt_result *res) {
+    if (!p || !res) return fail(QAP_PT_E_INVAL, "NULL argument");
+    if (inst < 0 || inst >= p->count) return fail(QAP_PT_E_INVAL, "instance %d out of range", inst);
+    HIP_TRY(hipSetDevice(p->device));
+    const InstDesc &d = p->desc[inst];
+    const int RT = d.R * d.T, n = d.n;
+    std::vector<int64_t> E(RT), bestE(RT);
+    std::vector<uint64_t> acc(RT);
+    InstState st;
+    HIP_TRY(hipMemcpy(E.data(), p->d_E + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(bestE.data(), p->d_bestE + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(acc.data(), p->d_accepted + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&st, p->d_state + inst, sizeof(st), hipMemcpyDeviceToHost));
+    int bi = -1;
+    int64_t best = INT64_MAX;
+    for (int r = 0; r < RT; ++r)
+        if (bestE[r] < best) {
+            best = bestE[r];
+            bi = r;
+        }
+    if (bi < 0) {  // no accepted move anywhere: the best initial state
+        for (int r = 0; r < RT; ++r)
+            if (bi < 0 || E[r] < best) {
+                best = E[r];
+                bi = r;
+            }
+    }
+    res->best_cost = best;
+    if (res->best_perm)
+        HIP_TRY(hipMemcpy(res->best_perm, p->d_best_perm + d.p_base + (int64_t)bi * n, 4 * n,
+                          hipMemcpyDeviceToHost));
+    res->sweeps_done = st.megas_done * d.spe;
+    res->attempts = (uint64_t)res->sweeps_done * (uint64_t)n * (uint64_t)RT;
+    res->accepted = 0;
+    for (auto a : acc) res->accepted += a;
+    res->exchanges = st.exchanges;
+    res->kicks = st.kicks;
+    res->seconds = p->seconds;
+    return QAP_PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *qap_pt_last_error(void) { return g_last_error.c_str(); }
+
+int qap_pt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int qap_pt_pool_create(int32_t count, const qap_problem *problems, const qap_pt_params *params,
+                       qap_pt_pool **out) {
+    try {
+        return pool_create_impl(count, problems, params, out);
+    } catch (const std::bad_alloc &) {
+        return fail(QAP_PT_E_NOMEM, "host allocation failed");
+    } catch (...) {
+        return fail(QAP_PT_E_INVAL, "unexpected host exception");
+    }
+}
+
+int qap_pt_pool_run(qap_pt_pool *pool, int64_t sweeps, double *kernel_ms, int64_t *launches) {
+    try {
+        return pool_run_impl(pool, sweeps, kernel_ms, launches);
+    } catch (...) {
+        return fail(QAP_PT_E_NOMEM, "host exception in run");
+    }
+}
+
+int qap_pt_pool_result(qap_pt_pool *pool, int32_t inst, qap_pt_result *res) {
+    try {
+        return pool_result_impl(pool, inst, res);
+    } catch (...) {
+        return fail(QAP_PT_E_NOMEM, "host exception in result");
+    }
+}
+
+int qap_pt_pool_state(qap_pt_pool *p, int32_t inst, int32_t *perm, int64_t *E, int64_t *bestE,
+                      int32_t *pos, uint64_t *accepted, int32_t *best_perms) {
+    if (!p) return fail(QAP_PT_E_INVAL, "pool is NULL");
+    if (inst < 0 || inst >= p->count) return fail(QAP_PT_E_INVAL, "instance %d out of range", inst);
+    HIP_TRY(hipSetDevice(p->device));
+    const InstDesc &d = p->desc[inst];
+    const int64_t RT = (int64_t)d.R * d.T, n = d.n;
+    if (perm)
+        HIP_TRY(hipMemcpy(perm, p->d_perm + d.p_base, 4 * RT * n, hipMemcpyDeviceToHost));
+    if (best_perms)
+        HIP_TRY(hipMemcpy(best_perms, p->d_best_perm + d.p_base, 4 * RT * n, hipMemcpyDeviceToHost));
+    if (E) HIP_TRY(hipMemcpy(E, p->d_E + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
+    if (bestE) HIP_TRY(hipMemcpy(bestE, p->d_bestE + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
+    if (pos) HIP_TRY(hipMemcpy(pos, p->d_pos + d.rep_base, 4 * RT, hipMemcpyDeviceToHost));
+    if (accepted)
+        HIP_TRY(hipMemcpy(accepted, p->d_accepted + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
+    return QAP_PT_OK;
+}
+
+int qap_pt_pool_verify(qap_pt_pool *p, int64_t *force_mismatches, int64_t *cost_mismatches) {
+    if (!p) return fail(QAP_PT_E_INVAL, "pool is NULL");
+    HIP_TRY(hipSetDevice(p->device));
+    int64_t fm = 0, cm = 0;
+    for (int k = 0; k < p->count; ++k) {
+        const InstDesc &d = p->desc[k];
+        const int RT = d.R * d.T;
+        int32_t *scratch = nullptr;
+        int64_t *Es = nullptr;
+        unsigned long long *cnt = nullptr;
+        const size_t words = (size_t)RT * d.n * d.ld;
+        if (hipMalloc((void **)&scratch, 4 * words) != hipSuccess ||
+            hipMalloc((void **)&Es, 8 * RT) != hipSuccess ||
+            hipMalloc((void **)&cnt, 8) != hipSuccess) {
+            if (scratch) (void)hipFree(scratch);
+            if (Es) (void)hipFree(Es);
+            return fail(QAP_PT_E_NOMEM, "verify scratch allocation failed");
+        }
+        (void)hipMemsetAsync(scratch, 0, 4 * words, p->stream);
+        (void)hipMemsetAsync(cnt, 0, 8, p->stream);
+        (void)launch_force(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n, d.ld, RT,
+                     scratch, p->stream);
+        (void)launch_energy(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n, RT, Es,
+                      p->stream);
+        (void)launch_count_mismatch(scratch, p->d_I + d.I_base, d.n, d.ld, (int64_t)RT * d.n, cnt,
+                              p->stream);
+        unsigned long long c = 0;
+        std::vector<int64_t> e1(RT), e2(RT);
+        hipError_t e = hipStreamSynchronize(p->stream);
+        if (e == hipSuccess) e = hipMemcpy(&c, cnt, 8, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(e1.data(), Es, 8 * RT, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(e2.data(), p->d_E + d.rep_base, 8 * RT, hipMemcpyDeviceToHost);
+        (void)hipFree(scratch);
+        (void)hipFree(Es);
+        (void)hipFree(cnt);
+        if (e != hipSuccess) return fail(QAP_PT_E_DEVICE, "verify failed: %s", hipGetErrorString(e));
+        fm += (int64_t)c;
+        for (int r = 0; r < RT; ++r) cm += e1[r] != e2[r];
+    }
+    if (force_mismatches) *force_mismatches = fm;
+    if (cost_mismatches) *cost_mismatches = cm;
+    return QAP_PT_OK;
+}
+
+int qap_pt_pool_destroy(qap_pt_pool *pool) {
+    pool_free(pool);
+    return QAP_PT_OK;
+}
+
+int qap_pt_solve_batch_hip(int32_t count, const qap_problem *problems,
+                           const qap_pt_params *params, qap_pt_result *results) {
+    if (!results) return fail(QAP_PT_E_INVAL, "results is NULL");
+    qap_pt_pool *p = nullptr;
+    int rc = qap_pt_pool_create(count, problems, params, &p);
+    if (rc) return rc;
+    rc = qap_pt_pool_run(p, -1, nullptr, nullptr);
+    for (int k = 0; rc == QAP_PT_OK && k < count; ++k) rc = qap_pt_pool_result(p, k, &results[k]);
+    pool_free(p);
+    return rc;
+}
+
+int qap_pt_solve_hip(const qap_problem *prob, const qap_pt_params *par, qap_pt_result *res) {
+    if (!prob || !par || !res) return fail(QAP_PT_E_INVAL, "NULL argument");
+    return qap_pt_solve_batch_hip(1, prob, par, res);
+}
+
+int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen) {
+    if (n < 1 || n > QAP_PT_MAX_N || !buf || buflen < 1) return fail(QAP_PT_E_INVAL, "bad argument");
+    const SweepShape s = choose_shape(n, true);
+    std::snprintf(buf, buflen, "%dx%d,%s", s.nt, s.cpl, s.lds_i ? "lds" : "hbm");
+    return QAP_PT_OK;
+}
+
+}  // extern "C"

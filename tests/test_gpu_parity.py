@@ -1,0 +1,145 @@
+"""GPU parity: the HIP sweep/exchange kernels against the CPU oracle, bit for bit.
+
+Every case runs the same seeded instance, ladder and Philox stream through the
+C-ABI (libqap_pt.so) and through oracle/qap_oracle.c, then compares the complete
+replica state: permutations, exact costs, per-replica bests and their permutations,
+ladder positions, accepted-move counts, exchange/kick counts, and the best result.
+The incrementally tracked force matrices and costs are also rebuilt from scratch on
+the device (qap_pt_pool_verify) and must agree exactly.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(qpt, oracle, F, D, betas, sweeps, T=1, spe=1, seed=7, kick_after=100, goal=None,
+              per_replica_init=False, env=None, init_perm=None):
+    env = env or {}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        inst = qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=sweeps, T=T, sweeps_per_exchange=spe,
+                            seed=seed, kick_after=kick_after, goal=goal,
+                            per_replica_init=per_replica_init, init_perm=init_perm)
+        with qpt.Pool([inst]) as pool:
+            pool.run(-1)
+            res = pool.result(0)
+            st = pool.state(0)
+            fm, cm = pool.verify()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    ref = oracle.solve(F, D, betas, sweeps, T=T, spe=spe, seed=seed, kick_after=kick_after,
+                       goal=goal, per_replica_init=per_replica_init, init_perm=init_perm)
+    return res, st, (fm, cm), ref
+
+
+def _assert_same(res, st, ver, ref):
+    assert ver == (0, 0), f"force/cost mismatches after the run: {ver}"
+    np.testing.assert_array_equal(st["E"], ref["final_E"])
+    np.testing.assert_array_equal(st["perm"], ref["final_perm"])
+    np.testing.assert_array_equal(st["bestE"], ref["final_bestE"])
+    np.testing.assert_array_equal(st["pos"], ref["final_pos"])
+    np.testing.assert_array_equal(st["accepted"], ref["final_accepted"])
+    np.testing.assert_array_equal(st["best_perms"], ref["best_perms"])
+    assert res["best_cost"] == ref["best_cost"]
+    np.testing.assert_array_equal(res["best_perm"], ref["best_perm"])
+    assert res["accepted"] == ref["accepted"]
+    assert res["exchanges"] == ref["exchanges"]
+    assert res["kicks"] == ref["kicks"]
+    assert res["attempts"] == ref["attempts"]
+
+
+CASES = {
+    # name: (instance builder, R, sweeps, extra kwargs)
+    "sym_n16_R4": (lambda q: q.instances.random_instance(11, 16), 4, 40, {}),
+    "c1_n64_R16": (lambda q: q.instances.flat_c1(), 16, 12, {}),
+    "sym_n100_R8": (lambda q: q.instances.mixed_instance(5, 100, True), 8, 6, {}),
+    "sparse_n150_R4": (lambda q: q.instances.mixed_instance(6, 150, False), 4, 4, {}),
+    "asym_diag_n24_T2_spe2": (lambda q: q.instances.random_instance(12, 24, symmetric=False,
+                                                                   zero_diag=False), 6, 20,
+                              {"T": 2, "spe": 2}),
+    "asym_n70_R5": (lambda q: q.instances.random_instance(13, 70, 0, 50, symmetric=False), 5, 6,
+                    {}),
+    "hbm_n40_R6": (lambda q: q.instances.random_instance(14, 40), 6, 10,
+                   {"env": {"QAP_PT_FORCE_HBM": "1"}}),
+    "per_replica_init_n33_T3": (lambda q: q.instances.random_instance(15, 33), 5, 10,
+                                {"T": 3, "per_replica_init": True}),
+    "kick_often_n20": (lambda q: q.instances.random_instance(16, 20), 4, 60, {"kick_after": 2}),
+    "multiwave_n64_256x1": (lambda q: q.instances.flat_c1(), 4, 6,
+                            {"env": {"QAP_PT_SHAPE": "256,1"}}),
+    "multiwave_n64_128x1": (lambda q: q.instances.flat_c1(), 4, 6,
+                            {"env": {"QAP_PT_SHAPE": "128,1"}}),
+    "cpl4_n200_64x4": (lambda q: q.instances.mixed_instance(17, 200, True), 3, 2,
+                       {"env": {"QAP_PT_SHAPE": "64,4"}}),
+    "grey_n256_R4": (lambda q: q.instances.grey_c2(), 4, 2, {}),
+    "n300_256x2_hbm": (lambda q: q.instances.mixed_instance(18, 300, True), 2, 1, {}),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_trajectory_bit_exact(qpt, oracle, gpu, name):
+    build, R, sweeps, kw = CASES[name]
+    F, D = build(qpt)
+    n = F.shape[0]
+    betas = qpt.instances.geometric_ladder(R, 1.0, 100.0) / max(1.0, n / 16.0)
+    print(f"case {name}: n={n} R={R} sweeps={sweeps} kw={kw}", flush=True)
+    res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, sweeps, **kw)
+    _assert_same(res, st, ver, ref)
+
+
+def test_goal_stops_like_oracle(qpt, oracle, gpu):
+    F, D = qpt.instances.random_instance(21, 30)
+    betas = qpt.instances.geometric_ladder(6, 1.0, 50.0)
+    free = oracle.solve(F, D, betas, 60, seed=3)
+    goal = int(free["best_cost"] * 1.02)
+    res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, 60, seed=3, goal=goal)
+    assert ref["megas_done"] < 60
+    assert res["sweeps_done"] == ref["megas_done"]
+    _assert_same(res, st, ver, ref)
+
+
+def test_incremental_runs_equal_one_run(qpt, oracle, gpu):
+    F, D = qpt.instances.random_instance(22, 48)
+    betas = qpt.instances.geometric_ladder(8, 1.0, 100.0)
+    inst = qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=30, seed=9)
+    with qpt.Pool([inst]) as pool:
+        pool.run(10)
+        pool.run(20)
+        st = pool.state(0)
+    ref = oracle.solve(F, D, betas, 30, seed=9)
+    np.testing.assert_array_equal(st["perm"], ref["final_perm"])
+    np.testing.assert_array_equal(st["E"], ref["final_E"])
+
+
+def test_batch_matches_single_and_oracle(qpt, oracle, gpu):
+    insts = []
+    for k, n in enumerate([17, 64, 90, 131, 40]):
+        F, D = qpt.instances.mixed_instance(100 + k, n, dense=(k % 2 == 0))
+        betas = qpt.instances.geometric_ladder(4 + k, 1.0, 100.0)
+        insts.append(qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=4 * (k + 1), seed=50 + k))
+    batch = qpt.solve_batch(insts)
+    for k, inst in enumerate(insts):
+        single = qpt.solve(inst)
+        ref = oracle.solve(inst.F, inst.D, inst.beta_ln, inst.sweeps, seed=inst.seed)
+        assert batch[k]["best_cost"] == single["best_cost"] == ref["best_cost"]
+        np.testing.assert_array_equal(batch[k]["best_perm"], ref["best_perm"])
+        assert batch[k]["accepted"] == ref["accepted"]
+
+
+def test_timed_mode_same_trajectory(qpt, oracle, gpu):
+    F, D = qpt.instances.flat_c1()
+    betas = qpt.instances.geometric_ladder(16, 1.0, 100.0)
+    inst = qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=8, seed=4)
+    with qpt.Pool([inst]) as pool:
+        ms, launches = pool.run(-1, timed=True)
+        st = pool.state(0)
+    assert launches == 8 and ms > 0
+    ref = oracle.solve(F, D, betas, 8, seed=4)
+    np.testing.assert_array_equal(st["perm"], ref["final_perm"])
