@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X parallel-tempering QAP annealer (one GPU).
+
+A step is one pass of the hot path over the workload: `--sweeps-per-step` sweeps of
+every replica (n p-dit updates = swap attempts each), with a replica exchange after
+every sweep.  W untimed steps, then K timed steps; the state (flow/distance,
+permutations, force matrices) is resident in HBM before the timed region starts.
+
+Prints one JSON line: swap-attempts/s (whole job), the roofline of the sweep kernel
+(algorithmic bytes per launch / mean launch time from HIP events on the launching
+stream), and a CPU baseline (the oracle's reference-faithful floating-point mode on
+the host cores, a bounded sample of the same workload).
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(
+    ROOT, "p-dit-probabilistic-ising-machine-for-solving-the-quadratic-assignment-problem_amd")
+
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    if "qap_pt_amd" in sys.modules:
+        return sys.modules["qap_pt_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "qap_pt_amd", os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["qap_pt_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def workload(pkg, name):
+    """Returns (instances, description dict, cpu sample spec)."""
+    I = pkg.instances
+    if name == "grey256":
+        F, D = I.grey_c2()
+        betas = I.geometric_ladder(64, 10.0, 1000.0)
+        inst = [pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=0, seed=2024)]
+        desc = {"workload": "grey-pattern n=256 (92x92 unit-flow block, squared torus "
+                            "Manhattan distance 16x16), 64 replicas, geometric T 10..1000, "
+                            "exchange every sweep", "n": 256, "replicas": 64, "instances": 1}
+        return inst, desc, {"ladders_per_thread": 1, "sweeps": 4}
+    if name == "flat64":
+        F, D = I.flat_c1()
+        betas = I.geometric_ladder(16, 1.0, 100.0)
+        inst = [pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=0, seed=2024)]
+        desc = {"workload": "n=64 uniform flow [0,99], rounded Euclidean distance, 16 "
+                            "replicas, geometric T 1..100, exchange every sweep", "n": 64,
+                "replicas": 16, "instances": 1}
+        return inst, desc, {"ladders_per_thread": 1, "sweeps": 4000}
+    if name == "batch512":
+        insts = []
+        for k, (F, D) in enumerate(I.batch_c3()):
+            betas = I.geometric_ladder(32, 100.0, 10000.0)
+            insts.append(pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=0, seed=7000 + k))
+        desc = {"workload": "512 instances, n uniform [30,150], half dense / half sparse "
+                            "flow, Euclidean distance, 32 replicas each, geometric T "
+                            "100..10000, exchange every sweep, one launch per step",
+                "n": "30..150", "replicas": 32 * 512, "instances": 512}
+        return insts, desc, {"ladders_per_thread": 1, "sweeps": 4}
+    if name in ("large512", "large1024"):
+        n = 512 if name == "large512" else 1024
+        F, D = I.large_c5(n)
+        betas = I.geometric_ladder(64, 300.0, 30000.0)
+        inst = [pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=0, seed=2024)]
+        desc = {"workload": f"n={n} flow U[0,255], Manhattan grid distance, 64 replicas, "
+                            "geometric T 300..30000, exchange every sweep", "n": n,
+                "replicas": 64, "instances": 1}
+        return inst, desc, {"ladders_per_thread": 1, "sweeps": 1}
+    raise SystemExit(f"unknown workload {name}")
+
+
+def algorithmic_bytes(attempts, n_weighted_attempts, accepted_n, cells):
+    """24 n bytes per attempt (six int32 n-vectors: force row, gathered force column,
+    gathered force diagonal, distance row, flow row, inverse permutation) plus, per
+    accepted move, 8 bytes per rewritten force entry (read + write) and 16 n bytes for
+    the two difference vectors."""
+    return 24 * n_weighted_attempts + 8 * cells + 16 * accepted_n
+
+
+def cpu_baseline(pkg, insts, spec, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    inst = insts[0]
+    sweeps = spec["sweeps"]
+    t0 = time.perf_counter()
+    r = pyoracle.solve(inst.F, inst.D, inst.beta_ln, sweeps, T=threads, seed=inst.seed,
+                       mode=pyoracle.MODE_REFFP, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": r["attempts"] / dt,
+        "unit": "swap-attempts/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{threads} independent ladders x {inst.R} replicas x {sweeps} sweeps of "
+                   f"n={inst.n} on {threads} threads ({r['attempts']} attempts, {dt:.1f} s); "
+                   "reference floating-point roulette (long double exp2, double sums) and "
+                   "full n^2 force update per accepted move as CPU_PIM.cpp:195-290"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="grey256",
+                    choices=["grey256", "flat64", "batch512", "large512", "large1024"])
+    ap.add_argument("--sweeps-per-step", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: every instance and ladder "
+                                   "is independent, multi-GPU sharding is described in "
+                                   "DESIGN.md but not built; rerun with --gpus 1"}))
+        sys.exit(2)
+
+    pkg = load_pkg()
+    insts, desc, cpu_spec = workload(pkg, args.workload)
+    sps = args.sweeps_per_step or {"grey256": 100, "flat64": 200, "batch512": 10,
+                                   "large512": 10, "large1024": 4}[args.workload]
+
+    pool = pkg.Pool(insts)
+    if args.warmup:
+        pool.run(args.warmup * sps)
+    before = [pool.result(k) for k in range(len(insts))]
+    t0 = time.perf_counter()
+    kernel_ms, launches = pool.run(args.steps * sps, timed=True)
+    t1 = time.perf_counter()
+    after = [pool.result(k) for k in range(len(insts))]
+    fm, cm = pool.verify()
+    pool.close()
+
+    wall = t1 - t0
+    attempts = sum(a["attempts"] - b["attempts"] for a, b in zip(after, before))
+    n_att = sum((a["attempts"] - b["attempts"]) * x.n for a, b, x in zip(after, before, insts))
+    acc = sum(a["accepted"] - b["accepted"] for a, b in zip(after, before))
+    acc_n = sum((a["accepted"] - b["accepted"]) * x.n for a, b, x in zip(after, before, insts))
+    cells = sum(a["update_cells"] - b["update_cells"] for a, b in zip(after, before))
+    alg = algorithmic_bytes(attempts, n_att, acc_n, cells)
+    avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
+    achieved = alg / max(launches, 1) / avg_launch_s / 1e9
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "swap-attempts/s",
+        "value": attempts / wall,
+        "unit": "swap-attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": dict(desc, sweeps_per_step=sps, kernel_shape=pkg.kernel_shape(insts[0].n)),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "accept_rate": acc / max(attempts, 1),
+        "sweep_kernel_avg_ms": kernel_ms / max(launches, 1),
+        "sweep_kernel_launches": launches,
+        "verify_mismatches": fm + cm,
+        "best_cost": min(a["best_cost"] for a in after),
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(pkg, insts, cpu_spec, threads)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

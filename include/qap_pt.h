@@ -79,6 +79,8 @@ typedef struct {
     uint64_t accepted;           /* updates that moved a facility                          */
     uint64_t exchanges;          /* ladder swaps by the exchange pass                     */
     uint64_t kicks;              /* hot/cold kicks                                         */
+    uint64_t update_cells;       /* force-matrix entries rewritten by rank-2 updates
+                                    (columns with a zero flow difference are skipped)      */
     int64_t sweeps_done;         /* per replica                                            */
     double seconds;              /* wall time of the sweep phase                          */
 } qap_pt_result;

@@ -79,6 +79,7 @@ class QapPtResult(C.Structure):
         ("accepted", C.c_uint64),
         ("exchanges", C.c_uint64),
         ("kicks", C.c_uint64),
+        ("update_cells", C.c_uint64),
         ("sweeps_done", C.c_int64),
         ("seconds", C.c_double),
     ]
@@ -198,6 +199,7 @@ def _result_dict(res: QapPtResult, perm: np.ndarray) -> dict:
         "accepted": int(res.accepted),
         "exchanges": int(res.exchanges),
         "kicks": int(res.kicks),
+        "update_cells": int(res.update_cells),
         "sweeps_done": int(res.sweeps_done),
         "seconds": float(res.seconds),
     }

@@ -117,6 +117,7 @@ struct PoolDev {
     int32_t *perm, *pinv, *best_perm, *I;
     int64_t *E, *bestE;
     uint64_t *accepted;
+    uint64_t *cells;          // force-matrix entries rewritten by rank-2 updates
     int32_t *pos;             // ladder position of each replica
     int32_t *order;           // per ladder: position -> replica (local index)
     int32_t *tss;             // per ladder: timeSinceSwap

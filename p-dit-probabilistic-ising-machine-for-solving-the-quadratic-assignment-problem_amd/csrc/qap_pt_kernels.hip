@@ -134,6 +134,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     int64_t E = P.E[g];
     int64_t bestE = P.bestE[g];
     uint64_t acc = P.accepted[g];
+    uint64_t cells = P.cells[g];
     const int pos = P.pos[g];
     const float beta2f = P.beta2f[d.beta_off + pos];
     const double c20 = P.c20[d.beta_off + pos];
@@ -274,6 +275,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                 }
             }
             __syncthreads();
+            cells += (uint64_t)n * nnz;
             if (nnz > 0) {
                 // 2-D split: CW threads across the compacted columns, NT/CW rows at once.
                 int cw = 1;
@@ -322,6 +324,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         P.E[g] = E;
         P.bestE[g] = bestE;
         P.accepted[g] = acc;
+        P.cells[g] = cells;
     }
 }
 

@@ -175,7 +175,7 @@ struct qap_pt_pool {
     double *d_c20 = nullptr;
     int32_t *d_perm = nullptr, *d_pinv = nullptr, *d_best_perm = nullptr, *d_I = nullptr;
     int64_t *d_E = nullptr, *d_bestE = nullptr, *d_limits = nullptr;
-    uint64_t *d_accepted = nullptr;
+    uint64_t *d_accepted = nullptr, *d_cells = nullptr;
     int32_t *d_pos = nullptr, *d_order = nullptr, *d_tss = nullptr;
     int64_t *d_ladder_best = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -238,7 +238,7 @@ void pool_free(qap_pt_pool *p) {
     void *bufs[] = {p->d_inst,  p->d_state,     p->d_rep_inst, p->d_F,      p->d_D,
                     p->d_Ft,    p->d_Dt,        p->d_Fdiag,    p->d_Ddiag,  p->d_beta2f,
                     p->d_c20,   p->d_perm,      p->d_pinv,     p->d_best_perm, p->d_I,
-                    p->d_E,     p->d_bestE,     p->d_limits,   p->d_accepted, p->d_pos,
+                    p->d_E,     p->d_bestE,     p->d_limits,   p->d_accepted, p->d_cells, p->d_pos,
                     p->d_order, p->d_tss,       p->d_ladder_best};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -421,6 +421,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(dalloc(&p->d_E, RTall));
     POOL_TRY(dalloc(&p->d_bestE, RTall));
     POOL_TRY(dalloc(&p->d_accepted, RTall));
+    POOL_TRY(dalloc(&p->d_cells, RTall));
     POOL_TRY(dalloc(&p->d_pos, RTall));
     POOL_TRY(dalloc(&p->d_order, RTall));
     POOL_TRY(dalloc(&p->d_tss, p->total_ladders));
@@ -458,6 +459,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(up(p->d_best_perm, perm.data(), 4 * perm.size()));
     POOL_TRY(up(p->d_bestE, bestE.data(), 8 * bestE.size()));
     POOL_TRY(up(p->d_accepted, acc.data(), 8 * acc.size()));
+    POOL_TRY(up(p->d_cells, acc.data(), 8 * acc.size()));
     POOL_TRY(up(p->d_pos, pos.data(), 4 * pos.size()));
     POOL_TRY(up(p->d_order, order.data(), 4 * order.size()));
     POOL_TRY(up(p->d_tss, tss.data(), 4 * tss.size()));
@@ -483,6 +485,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     P.E = p->d_E;
     P.bestE = p->d_bestE;
     P.accepted = p->d_accepted;
+    P.cells = p->d_cells;
     P.pos = p->d_pos;
     P.order = p->d_order;
     P.tss = p->d_tss;
@@ -623,11 +626,12 @@ int pool_result_impl(qap_pt_pool *p, int inst, qap_pt_result *res) {
     const InstDesc &d = p->desc[inst];
     const int RT = d.R * d.T, n = d.n;
     std::vector<int64_t> E(RT), bestE(RT);
-    std::vector<uint64_t> acc(RT);
+    std::vector<uint64_t> acc(RT), cells(RT);
     InstState st;
     HIP_TRY(hipMemcpy(E.data(), p->d_E + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(bestE.data(), p->d_bestE + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(acc.data(), p->d_accepted + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cells.data(), p->d_cells + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(&st, p->d_state + inst, sizeof(st), hipMemcpyDeviceToHost));
     int bi = -1;
     int64_t best = INT64_MAX;
@@ -651,6 +655,8 @@ int pool_result_impl(qap_pt_pool *p, int inst, qap_pt_result *res) {
     res->attempts = (uint64_t)res->sweeps_done * (uint64_t)n * (uint64_t)RT;
     res->accepted = 0;
     for (auto a : acc) res->accepted += a;
+    res->update_cells = 0;
+    for (auto c : cells) res->update_cells += c;
     res->exchanges = st.exchanges;
     res->kicks = st.kicks;
     res->seconds = p->seconds;
