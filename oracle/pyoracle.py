@@ -80,6 +80,9 @@ def lib() -> C.CDLL:
         L.oracle_solve.argtypes = [C.POINTER(_Problem), C.POINTER(_Params), C.POINTER(_Result),
                                    C.c_int32, C.c_int32]
         L.oracle_bounds.argtypes = [C.c_int32, _I32P, _I32P, _I64P, _I64P]
+        L.oracle_exchange_step.argtypes = [C.c_int32, _I64P, _I64P, _I32P, _I32P, _I32P, _I64P,
+                                           C.c_int32, _I32P]
+        L.oracle_exchange_step.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -235,3 +238,19 @@ def solve(F, D, beta_ln, sweeps, R=None, T=1, spe=1, seed=0, kick_after=100, goa
     if traj:
         out["traj_E"] = traj_E
     return out
+
+
+def exchange_step(E, bestE, order, pos, tss, ladder_best, kick_after=100):
+    """One exchange step (CPU_PIM.cpp:295-344) on copies; returns the new state."""
+    E = np.ascontiguousarray(E, np.int64)
+    bE = np.ascontiguousarray(bestE, np.int64)
+    order = np.ascontiguousarray(order, np.int32).copy()
+    pos = np.ascontiguousarray(pos, np.int32).copy()
+    t = C.c_int32(int(tss))
+    lb = C.c_int64(int(ladder_best))
+    kicked = C.c_int32(0)
+    nex = lib().oracle_exchange_step(E.size, _p(E, C.c_int64), _p(bE, C.c_int64),
+                                     _p(order, C.c_int32), _p(pos, C.c_int32), C.byref(t),
+                                     C.byref(lb), int(kick_after), C.byref(kicked))
+    return {"order": order, "pos": pos, "tss": int(t.value), "ladder_best": int(lb.value),
+            "exchanges": int(nex), "kicked": bool(kicked.value)}

@@ -290,40 +290,62 @@ typedef struct {
     int32_t *dE;
     uint32_t *w;
     long double *probs;
+    int64_t *xE, *xbE;      /* R scratch for the exchange step */
+    int32_t *xpos;
     int64_t *traj;          /* traj_E base (may be NULL) */
     int64_t traj_cap;
     int32_t RT;
 } ladder_t;
 
-/* Fold + exchange + kick (CPU_PIM.cpp:295-344). */
+int32_t oracle_exchange_step(int32_t R, const int64_t *E, const int64_t *bestE, int32_t *order,
+                             int32_t *pos, int32_t *tss, int64_t *ladder_best,
+                             int32_t kick_after, int32_t *kicked) {
+    int32_t nex = 0;
+    (*tss)++;                                   /* CPU_PIM.cpp:295 */
+    for (int l = 0; l < R; l++) {               /* :297-314 */
+        if (bestE[l] < *ladder_best) {
+            *ladder_best = bestE[l];
+            *tss = 0;
+        }
+    }
+    for (int k = 0; k < R - 1; k++) {           /* :324-337, hottest first, bubbling */
+        int32_t r1 = order[k], r2 = order[k + 1];
+        if (E[r1] < E[r2]) {
+            order[k] = r2;
+            order[k + 1] = r1;
+            pos[r1] = k + 1;                    /* the beta swap of :332 */
+            pos[r2] = k;
+            nex++;
+            if (r2 == R - 1) *tss = 0;          /* :333-335 tests a replica index */
+        }
+    }
+    *kicked = 0;
+    if (*tss > kick_after) {                    /* :339-344, not reset by the kick */
+        int32_t r1 = order[R - 1], r2 = order[0];
+        order[R - 1] = r2;
+        order[0] = r1;
+        pos[r1] = 0;
+        pos[r2] = R - 1;
+        *kicked = 1;
+    }
+    return nex;
+}
+
+/* Fold + exchange + kick of one ladder. */
 static void ladder_exchange(ladder_t *L) {
     const int32_t R = L->par->R;
-    L->tss++;
+    int64_t *E = L->xE, *bE = L->xbE;
+    int32_t *pos = L->xpos;
     for (int l = 0; l < R; l++) {
-        if (L->reps[l].bestE < L->ladderBest) {
-            L->ladderBest = L->reps[l].bestE;
-            L->tss = 0;
-        }
+        E[l] = L->reps[l].E;
+        bE[l] = L->reps[l].bestE;
+        pos[l] = L->reps[l].pos;
     }
-    for (int k = 0; k < R - 1; k++) {
-        int32_t r1 = L->order[k], r2 = L->order[k + 1];
-        if (L->reps[r1].E < L->reps[r2].E) {
-            L->order[k] = r2;
-            L->order[k + 1] = r1;
-            L->reps[r1].pos = k + 1;
-            L->reps[r2].pos = k;
-            L->exchanges++;
-            if (r2 == R - 1) L->tss = 0;
-        }
-    }
-    if (L->tss > L->par->kick_after) {
-        int32_t r1 = L->order[R - 1], r2 = L->order[0];
-        L->order[R - 1] = r2;
-        L->order[0] = r1;
-        L->reps[r1].pos = 0;
-        L->reps[r2].pos = R - 1;
-        L->kicks++;
-    }
+    int32_t kicked = 0;
+    L->exchanges += (uint64_t)oracle_exchange_step(R, E, bE, L->order, pos, &L->tss,
+                                                   &L->ladderBest, L->par->kick_after, &kicked);
+    L->kicks += (uint64_t)kicked;
+    for (int l = 0; l < R; l++) L->reps[l].pos = pos[l];
 }
 
 static void ladder_sweeps(ladder_t *L, int64_t m) {
@@ -445,6 +467,9 @@ int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_re
         L->dE = malloc(sizeof(int32_t) * n);
         L->w = malloc(sizeof(uint32_t) * n);
         L->probs = malloc(sizeof(long double) * n);
+        L->xE = malloc(sizeof(int64_t) * R);
+        L->xbE = malloc(sizeof(int64_t) * R);
+        L->xpos = malloc(sizeof(int32_t) * R);
         L->traj = res->traj_E;
         L->traj_cap = res->traj_E ? res->traj_cap : 0;
         L->RT = RT;
@@ -526,6 +551,9 @@ int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_re
         free(lad[t].dE);
         free(lad[t].w);
         free(lad[t].probs);
+        free(lad[t].xE);
+        free(lad[t].xbE);
+        free(lad[t].xpos);
     }
     for (int r = 0; r < RT; r++) {
         free(reps[r].perm);

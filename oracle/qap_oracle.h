@@ -108,6 +108,13 @@ int32_t oracle_attempt_exact(int32_t n, const int32_t *F, const int32_t *D, int3
                              int32_t *pinv, int32_t *I, int64_t *E, int32_t i, float beta2f,
                              double c20, uint32_t u, int32_t *dE, uint32_t *w);
 
+/* One exchange step of one ladder (CPU_PIM.cpp:295-344): best fold, bubble pass from
+ * the hottest position, hot/cold kick.  E/bestE/pos are indexed by replica, order by
+ * ladder position.  Returns the number of exchanges; *kicked is set to 0/1. */
+int32_t oracle_exchange_step(int32_t R, const int64_t *E, const int64_t *bestE, int32_t *order,
+                             int32_t *pos, int32_t *tss, int64_t *ladder_best,
+                             int32_t kick_after, int32_t *kicked);
+
 /* beta (natural-log units) -> exp2 units with the reference's constant
  * (CPU_PIM.cpp:528: "* 1.44269504089"). */
 double oracle_beta2(double beta_ln);

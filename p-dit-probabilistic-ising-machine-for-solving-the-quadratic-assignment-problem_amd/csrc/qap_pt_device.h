@@ -78,7 +78,9 @@ __host__ __device__ inline int32_t heat_cutoff(int32_t maxd, double c20) {
 // Descriptors (host writes, device reads).
 // ---------------------------------------------------------------------------
 struct InstDesc {
-    int32_t n, ld;            // size, row stride of the force matrix (elements)
+    int32_t n, ld;            // size, row stride of the force matrix in HBM (elements)
+    int32_t ld_s;             // row stride of the LDS copy (elements, odd word stride)
+    int32_t sparse_upd;       // compact the nonzero flow-difference columns on update
     int32_t R, T;             // replicas per ladder, ladders
     int32_t spe;              // sweeps per exchange
     int32_t kick_after;
@@ -112,6 +114,7 @@ struct PoolDev {
     const int32_t *rep_inst;  // global replica -> instance
     const int32_t *F, *D, *Ft, *Dt;
     const int32_t *Fdiag, *Ddiag;
+    const int32_t *fclass;    // per facility: equal id <=> identical flow row and column
     const float *beta2f;      // per ladder position (exp2 units, float)
     const double *c20;        // per ladder position: 20 / beta2 (double)
     int32_t *perm, *pinv, *best_perm, *I;

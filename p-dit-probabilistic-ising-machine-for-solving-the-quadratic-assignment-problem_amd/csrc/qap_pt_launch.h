@@ -11,15 +11,22 @@ namespace qap {
 // LDS available to one workgroup on gfx950 (160 KiB per CU).
 constexpr int kMaxLdsBytes = 160 * 1024;
 
+// Where a replica's force matrix lives during a sweep launch.
+enum IMode : int {
+    IM_LDS32 = 0,  // int32 in LDS
+    IM_LDS16 = 1,  // uint16 holding -I in LDS (F, D >= 0 and bound <= 65535)
+    IM_HBM = 2,    // int32 in HBM, updated in place
+};
+
 struct SweepShape {
     int nt;      // threads per workgroup (multiple of 64)
     int cpl;     // candidates per thread: n <= nt * cpl
-    bool lds_i;  // force matrix staged in LDS for the launch
+    int im;      // IMode
     bool sym;    // symmetric, zero-diagonal F and D
 };
 
 bool sweep_shape_supported(int nt, int cpl);
-int sweep_lds_bytes(int n, int ld, bool lds_i);
+int sweep_lds_bytes(int n, int ld_s, int im);
 
 hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,
                         int lds_bytes, hipStream_t s);

@@ -91,15 +91,26 @@ bool is_sym_zero_diag(int n, const int32_t *M) {
 
 int round4i(int x) { return (x + 3) & ~3; }
 
-SweepShape choose_shape(int n, bool sym) {
+// Odd word stride for the LDS copy of I: the column gather I[pinv[b]][a] then spreads
+// over all LDS banks.  For uint16 storage the stride in 32-bit words is ld_s / 2.
+int lds_stride(int n, int im) {
+    if (im == IM_LDS32) return (n % 2) ? n : n + 1;
+    int ld = (n + 1) & ~1;
+    if ((ld / 2) % 2 == 0) ld += 2;
+    return ld;
+}
+
+// Threads x candidates per thread and force-matrix placement for size n.
+// QAP_PT_SHAPE="nt,cpl" and QAP_PT_IMODE=0|1|2 override (tuning and tests).
+SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, int *ld_s) {
     SweepShape s{};
     s.sym = sym;
     if (n <= 64) {
         s.nt = 64;
         s.cpl = 1;
     } else if (n <= 128) {
-        s.nt = 64;
-        s.cpl = 2;
+        s.nt = 128;
+        s.cpl = 1;
     } else if (n <= 256) {
         s.nt = 256;
         s.cpl = 1;
@@ -110,7 +121,6 @@ SweepShape choose_shape(int n, bool sym) {
         s.nt = 256;
         s.cpl = 4;
     }
-    // Optional override for tuning sweeps: QAP_PT_SHAPE="nt,cpl" (must cover n).
     if (const char *env = std::getenv("QAP_PT_SHAPE")) {
         int nt = 0, cpl = 0;
         if (std::sscanf(env, "%d,%d", &nt, &cpl) == 2 && sweep_shape_supported(nt, cpl) &&
@@ -119,11 +129,46 @@ SweepShape choose_shape(int n, bool sym) {
             s.cpl = cpl;
         }
     }
-    const int ld = round4i(n);
-    s.lds_i = sweep_lds_bytes(n, ld, true) <= kMaxLdsBytes;
-    if (const char *env = std::getenv("QAP_PT_FORCE_HBM"))
-        if (env[0] == '1') s.lds_i = false;
+    const bool fits16 = nonneg && force_bound <= 65535;
+    if (sweep_lds_bytes(n, lds_stride(n, IM_LDS32), IM_LDS32) <= kMaxLdsBytes) s.im = IM_LDS32;
+    else if (fits16 && sweep_lds_bytes(n, lds_stride(n, IM_LDS16), IM_LDS16) <= kMaxLdsBytes)
+        s.im = IM_LDS16;
+    else s.im = IM_HBM;
+    if (const char *env = std::getenv("QAP_PT_IMODE")) {
+        const int im = std::atoi(env);
+        if (im == IM_HBM) s.im = IM_HBM;
+        if (im == IM_LDS16 && fits16 &&
+            sweep_lds_bytes(n, lds_stride(n, IM_LDS16), IM_LDS16) <= kMaxLdsBytes)
+            s.im = IM_LDS16;
+    }
+    *ld_s = lds_stride(n, s.im);
     return s;
+}
+
+// Facility classes: equal id <=> identical flow column and row, i.e. swapping the two
+// facilities leaves every flow difference v1, v2 of the rank-2 update zero.
+std::vector<int32_t> facility_classes(int n, const int32_t *F) {
+    std::map<std::vector<int32_t>, int32_t> ids;
+    std::vector<int32_t> cls(n);
+    std::vector<int32_t> key(2 * (size_t)n);
+    for (int a = 0; a < n; ++a) {
+        for (int c = 0; c < n; ++c) {
+            key[c] = F[(int64_t)c * n + a];
+            key[n + c] = F[(int64_t)a * n + c];
+        }
+        auto it = ids.find(key);
+        if (it == ids.end()) it = ids.emplace(key, (int32_t)ids.size()).first;
+        cls[a] = it->second;
+    }
+    return cls;
+}
+
+double offdiag_density(int n, const int32_t *F) {
+    if (n < 2) return 1.0;
+    int64_t nz = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) nz += (i != j) && F[(int64_t)i * n + j] != 0;
+    return (double)nz / ((double)n * (n - 1));
 }
 
 // Initial permutation (CPU_PIM.cpp:134-161): identity, then for i = 0..n-1 swap
@@ -170,7 +215,7 @@ struct qap_pt_pool {
     InstDesc *d_inst = nullptr;
     InstState *d_state = nullptr;
     int32_t *d_rep_inst = nullptr, *d_F = nullptr, *d_D = nullptr, *d_Ft = nullptr, *d_Dt = nullptr;
-    int32_t *d_Fdiag = nullptr, *d_Ddiag = nullptr;
+    int32_t *d_Fdiag = nullptr, *d_Ddiag = nullptr, *d_fclass = nullptr;
     float *d_beta2f = nullptr;
     double *d_c20 = nullptr;
     int32_t *d_perm = nullptr, *d_pinv = nullptr, *d_best_perm = nullptr, *d_I = nullptr;
@@ -178,6 +223,9 @@ struct qap_pt_pool {
     uint64_t *d_accepted = nullptr, *d_cells = nullptr;
     int32_t *d_pos = nullptr, *d_order = nullptr, *d_tss = nullptr;
     int64_t *d_ladder_best = nullptr;
+    std::vector<hipStream_t> side;     // one per extra bucket (default priority)
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_depth = 0;
@@ -235,17 +283,23 @@ void pool_free(qap_pt_pool *p) {
     if (p->graph) (void)hipGraphDestroy(p->graph);
     for (auto &b : p->buckets)
         if (b.d_list) (void)hipFree(b.d_list);
-    void *bufs[] = {p->d_inst,  p->d_state,     p->d_rep_inst, p->d_F,      p->d_D,
+    void *bufs[] = {p->d_inst,  p->d_state,     p->d_rep_inst, p->d_F,      p->d_D, p->d_fclass,
                     p->d_Ft,    p->d_Dt,        p->d_Fdiag,    p->d_Ddiag,  p->d_beta2f,
                     p->d_c20,   p->d_perm,      p->d_pinv,     p->d_best_perm, p->d_I,
                     p->d_E,     p->d_bestE,     p->d_limits,   p->d_accepted, p->d_cells, p->d_pos,
                     p->d_order, p->d_tss,       p->d_ladder_best};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    for (auto st : p->side) (void)hipStreamDestroy(st);
+    for (auto ev : p->ev_join) (void)hipEventDestroy(ev);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
 
+// One graph = `depth` mega-iterations: exchange, then every sweep bucket.  Buckets
+// (different kernel instantiations) run concurrently on side streams that fork from
+// and join back into the pool stream.
 int build_graph(qap_pt_pool *p, int depth) {
     if (p->graph_exec) {
         (void)hipGraphExecDestroy(p->graph_exec);
@@ -255,20 +309,41 @@ int build_graph(qap_pt_pool *p, int depth) {
         (void)hipGraphDestroy(p->graph);
         p->graph = nullptr;
     }
+    const int nb = (int)p->buckets.size();
+    while ((int)p->side.size() < nb - 1) {
+        hipStream_t st = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        p->side.push_back(st);
+        hipEvent_t ev = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        p->ev_join.push_back(ev);
+    }
+    if (nb > 1 && !p->ev_fork) HIP_TRY(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
     HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-    for (int m = 0; m < depth; ++m) {
-        hipError_t e = launch_exchange(p->P, p->count, p->stream);
-        for (auto &b : p->buckets) {
-            if (e != hipSuccess) break;
-            e = launch_sweep(p->P, b.shape, b.d_list, (int)b.reps.size(), b.lds_bytes, p->stream);
+    hipError_t e = hipSuccess;
+    for (int m = 0; m < depth && e == hipSuccess; ++m) {
+        e = launch_exchange(p->P, p->count, p->stream);
+        if (e == hipSuccess && nb > 1) e = hipEventRecord(p->ev_fork, p->stream);
+        for (int b = 1; b < nb && e == hipSuccess; ++b) {
+            e = hipStreamWaitEvent(p->side[b - 1], p->ev_fork, 0);
+            if (e == hipSuccess)
+                e = launch_sweep(p->P, p->buckets[b].shape, p->buckets[b].d_list,
+                                 (int)p->buckets[b].reps.size(), p->buckets[b].lds_bytes,
+                                 p->side[b - 1]);
+            if (e == hipSuccess) e = hipEventRecord(p->ev_join[b - 1], p->side[b - 1]);
         }
-        if (e != hipSuccess) {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(p->stream, &g);
-            if (g) (void)hipGraphDestroy(g);
-            return fail(QAP_PT_E_DEVICE, "kernel launch during capture failed: %s",
-                        hipGetErrorString(e));
-        }
+        if (e == hipSuccess && nb > 0)
+            e = launch_sweep(p->P, p->buckets[0].shape, p->buckets[0].d_list,
+                             (int)p->buckets[0].reps.size(), p->buckets[0].lds_bytes, p->stream);
+        for (int b = 1; b < nb && e == hipSuccess; ++b)
+            e = hipStreamWaitEvent(p->stream, p->ev_join[b - 1], 0);
+    }
+    if (e != hipSuccess) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(p->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return fail(QAP_PT_E_DEVICE, "kernel launch during capture failed: %s",
+                    hipGetErrorString(e));
     }
     HIP_TRY(hipStreamEndCapture(p->stream, &p->graph));
     HIP_TRY(hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0));
@@ -297,7 +372,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     p->shapes.resize(count);
     p->limits.assign(count, 0);
 
-    std::vector<int32_t> hF, hD, hFt, hDt, hFd, hDd, rep_inst, perm, pinv, pos, order;
+    std::vector<int32_t> hF, hD, hFt, hDt, hFd, hDd, hCls, rep_inst, perm, pinv, pos, order;
     std::vector<float> beta2f;
     std::vector<double> c20;
     std::map<std::tuple<int, int, int, int>, int> bucket_of;
@@ -307,10 +382,16 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
         const int n = pb.n;
         InstDesc &d = p->desc[k];
         const bool sym = is_sym_zero_diag(n, pb.F) && is_sym_zero_diag(n, pb.D);
-        const SweepShape sh = choose_shape(n, sym);
+        bool nonneg = true;
+        for (int64_t x = 0; x < (int64_t)n * n && nonneg; ++x) nonneg = pb.F[x] >= 0 && pb.D[x] >= 0;
+        const Bounds bnd = compute_bounds(n, pb.F, pb.D);
+        int ld_s = 0;
+        const SweepShape sh = choose_shape(n, sym, nonneg, bnd.force, &ld_s);
         p->shapes[k] = sh;
         d.n = n;
         d.ld = round4i(n);
+        d.ld_s = ld_s;
+        d.sparse_upd = offdiag_density(n, pb.F) < 0.35 ? 1 : 0;
         d.R = pr.R;
         d.T = pr.T;
         d.spe = pr.sweeps_per_exchange;
@@ -347,6 +428,8 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
             hFd.push_back(pb.F[(int64_t)i * n + i]);
             hDd.push_back(pb.D[(int64_t)i * n + i]);
         }
+        const std::vector<int32_t> cls = facility_classes(n, pb.F);
+        hCls.insert(hCls.end(), cls.begin(), cls.end());
         for (int l = 0; l < pr.R; ++l) {
             const double b2 = pr.beta_ln[l] * kLog2e;
             beta2f.push_back((float)b2);
@@ -372,7 +455,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
                 rep_inst.push_back(k);
             }
         }
-        const auto key = std::make_tuple(sh.nt, sh.cpl, (int)sh.lds_i, (int)sh.sym);
+        const auto key = std::make_tuple(sh.nt, sh.cpl, sh.im, (int)sh.sym);
         auto it = bucket_of.find(key);
         int bi;
         if (it == bucket_of.end()) {
@@ -384,8 +467,34 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
             bi = it->second;
         }
         Bucket &B = p->buckets[bi];
-        B.lds_bytes = std::max(B.lds_bytes, sweep_lds_bytes(n, d.ld, sh.lds_i));
+        B.lds_bytes = std::max(B.lds_bytes, sweep_lds_bytes(n, d.ld_s, sh.im));
         for (int r = 0; r < RT; ++r) B.reps.push_back(d.rep_base + r);
+    }
+
+    // Order every bucket's workgroups: larger n first (longest sweeps start first),
+    // and groups of 8 instances interleaved replica by replica so that consecutive
+    // workgroup ids (dispatched round-robin over the 8 XCDs) keep each instance on one
+    // XCD, where its F and D stay in that XCD's L2.
+    for (auto &B : p->buckets) {
+        std::map<int, std::vector<int32_t>> by_inst;
+        for (int32_t g : B.reps) by_inst[rep_inst[g]].push_back(g);
+        std::vector<int> insts;
+        for (auto &kv : by_inst) insts.push_back(kv.first);
+        std::stable_sort(insts.begin(), insts.end(), [&](int x, int y) {
+            return p->desc[x].n > p->desc[y].n;
+        });
+        std::vector<int32_t> order_list;
+        for (size_t g0 = 0; g0 < insts.size(); g0 += 8) {
+            const size_t g1 = std::min(insts.size(), g0 + 8);
+            size_t longest = 0;
+            for (size_t x = g0; x < g1; ++x) longest = std::max(longest, by_inst[insts[x]].size());
+            for (size_t r = 0; r < longest; ++r)
+                for (size_t x = g0; x < g1; ++x) {
+                    const auto &v = by_inst[insts[x]];
+                    if (r < v.size()) order_list.push_back(v[r]);
+                }
+        }
+        B.reps.swap(order_list);
     }
 
     auto bail = [&](int rc) {
@@ -412,6 +521,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(dalloc(&p->d_Dt, p->total_mat));
     POOL_TRY(dalloc(&p->d_Fdiag, p->total_vec));
     POOL_TRY(dalloc(&p->d_Ddiag, p->total_vec));
+    POOL_TRY(dalloc(&p->d_fclass, p->total_vec));
     POOL_TRY(dalloc(&p->d_beta2f, p->total_beta));
     POOL_TRY(dalloc(&p->d_c20, p->total_beta));
     POOL_TRY(dalloc(&p->d_perm, p->total_p));
@@ -452,6 +562,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(up(p->d_Dt, hDt.data(), 4 * hDt.size()));
     POOL_TRY(up(p->d_Fdiag, hFd.data(), 4 * hFd.size()));
     POOL_TRY(up(p->d_Ddiag, hDd.data(), 4 * hDd.size()));
+    POOL_TRY(up(p->d_fclass, hCls.data(), 4 * hCls.size()));
     POOL_TRY(up(p->d_beta2f, beta2f.data(), 4 * beta2f.size()));
     POOL_TRY(up(p->d_c20, c20.data(), 8 * c20.size()));
     POOL_TRY(up(p->d_perm, perm.data(), 4 * perm.size()));
@@ -476,6 +587,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     P.Dt = p->d_Dt;
     P.Fdiag = p->d_Fdiag;
     P.Ddiag = p->d_Ddiag;
+    P.fclass = p->d_fclass;
     P.beta2f = p->d_beta2f;
     P.c20 = p->d_c20;
     P.perm = p->d_perm;
@@ -789,8 +901,10 @@ int qap_pt_solve_hip(const qap_problem *prob, const qap_pt_params *par, qap_pt_r
 
 int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen) {
     if (n < 1 || n > QAP_PT_MAX_N || !buf || buflen < 1) return fail(QAP_PT_E_INVAL, "bad argument");
-    const SweepShape s = choose_shape(n, true);
-    std::snprintf(buf, buflen, "%dx%d,%s", s.nt, s.cpl, s.lds_i ? "lds" : "hbm");
+    int ld_s = 0;
+    const SweepShape s = choose_shape(n, true, true, (int64_t)1 << 40, &ld_s);
+    std::snprintf(buf, buflen, "%dx%d,%s", s.nt, s.cpl,
+                  s.im == IM_LDS32 ? "lds32" : (s.im == IM_LDS16 ? "lds16" : "hbm"));
     return QAP_PT_OK;
 }
 

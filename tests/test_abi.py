@@ -1,0 +1,63 @@
+"""The C-ABI library (libqap_pt.so): loads, exports every symbol include/qap_pt.h
+declares, and validates arguments on the host before touching a device -- CPU only."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "qap_pt.h")
+
+
+def _declared():
+    with open(HEADER) as fh:
+        txt = fh.read()
+    return sorted(set(re.findall(r"\b(qap_(?:pt|srv)_\w+)\s*\(", txt)))
+
+
+def test_header_declares_what_the_binding_lists(qpt):
+    assert _declared() == sorted(qpt.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol(qpt):
+    lib = qpt.load_library()
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts_match_header(qpt):
+    import ctypes as C
+    # field order and widths of the public structs (x86-64 SysV layout)
+    assert C.sizeof(qpt.QapProblem) == 24
+    assert qpt.QapPtParams.init_perm.offset == C.sizeof(qpt.QapPtParams) - 8
+    assert qpt.QapPtResult.seconds.offset == C.sizeof(qpt.QapPtResult) - 8
+
+
+def _expect(qpt, code, inst):
+    with pytest.raises(qpt.QapError) as ei:
+        qpt.Pool([inst])
+    assert ei.value.code == code, str(ei.value)
+
+
+def test_host_validation_errors(qpt):
+    F, D = qpt.instances.random_instance(1, 8)
+    ok = dict(F=F, D=D, beta_ln=np.array([0.1, 0.2]), sweeps=4)
+    _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(**dict(ok, beta_ln=np.array([0.2, 0.1]))))
+    _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(**dict(ok, beta_ln=np.array([0.0, 0.1]))))
+    _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(**dict(ok, sweeps=5, sweeps_per_exchange=2)))
+    _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(**dict(ok, init_perm=np.zeros(8, np.int32))))
+    big = np.full((8, 8), 1 << 20, dtype=np.int32)
+    np.fill_diagonal(big, 0)
+    _expect(qpt, qpt.QAP_PT_E_OVERFLOW, qpt.Instance(**dict(ok, F=big, D=big)))
+    Fz = np.zeros((QMAX := qpt.MAX_N + 1, QMAX), np.int32)
+    _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(**dict(ok, F=Fz, D=Fz)))
+
+
+def test_no_device_is_an_error_not_a_fallback(qpt):
+    """Without a visible GPU a valid request fails with QAP_PT_E_DEVICE: there is no
+    CPU path inside the product library."""
+    if qpt.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    F, D = qpt.instances.random_instance(2, 8)
+    _expect(qpt, qpt.QAP_PT_E_DEVICE, qpt.Instance(F=F, D=D, beta_ln=np.array([0.1]), sweeps=2))

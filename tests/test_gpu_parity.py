@@ -68,7 +68,24 @@ CASES = {
     "asym_n70_R5": (lambda q: q.instances.random_instance(13, 70, 0, 50, symmetric=False), 5, 6,
                     {}),
     "hbm_n40_R6": (lambda q: q.instances.random_instance(14, 40), 6, 10,
-                   {"env": {"QAP_PT_FORCE_HBM": "1"}}),
+                   {"env": {"QAP_PT_IMODE": "2"}}),
+    "hbm_asym_n37": (lambda q: q.instances.random_instance(19, 37, symmetric=False,
+                                                          zero_diag=False), 4, 8,
+                     {"env": {"QAP_PT_IMODE": "2"}}),
+    "lds16_n50_R6": (lambda q: q.instances.random_instance(20, 50), 6, 10,
+                     {"env": {"QAP_PT_IMODE": "1"}}),
+    "lds16_asym_n45": (lambda q: q.instances.random_instance(23, 45, symmetric=False,
+                                                            zero_diag=False), 4, 8,
+                       {"env": {"QAP_PT_IMODE": "1"}}),
+    "lds16_grey_n256_R4": (lambda q: q.instances.grey_c2(), 4, 3, {}),
+    "sparse_n90_64x2": (lambda q: q.instances.mixed_instance(24, 90, False), 5, 8,
+                        {"env": {"QAP_PT_SHAPE": "64,2"}}),
+    "sparse_n120_128x1_hbm": (lambda q: q.instances.mixed_instance(25, 120, False), 4, 6,
+                              {"env": {"QAP_PT_SHAPE": "128,1", "QAP_PT_IMODE": "2"}}),
+    "dense_n77_64x2": (lambda q: q.instances.mixed_instance(26, 77, True), 5, 8,
+                       {"env": {"QAP_PT_SHAPE": "64,2"}}),
+    "n1_trivial": (lambda q: q.instances.random_instance(27, 1), 3, 5, {}),
+    "n2_R2": (lambda q: q.instances.random_instance(28, 2), 2, 20, {}),
     "per_replica_init_n33_T3": (lambda q: q.instances.random_instance(15, 33), 5, 10,
                                 {"T": 3, "per_replica_init": True}),
     "kick_often_n20": (lambda q: q.instances.random_instance(16, 20), 4, 60, {"kick_after": 2}),
@@ -78,7 +95,7 @@ CASES = {
                             {"env": {"QAP_PT_SHAPE": "128,1"}}),
     "cpl4_n200_64x4": (lambda q: q.instances.mixed_instance(17, 200, True), 3, 2,
                        {"env": {"QAP_PT_SHAPE": "64,4"}}),
-    "grey_n256_R4": (lambda q: q.instances.grey_c2(), 4, 2, {}),
+    "grey_n256_R4_hbm": (lambda q: q.instances.grey_c2(), 4, 2, {"env": {"QAP_PT_IMODE": "2"}}),
     "n300_256x2_hbm": (lambda q: q.instances.mixed_instance(18, 300, True), 2, 1, {}),
 }
 
@@ -143,3 +160,21 @@ def test_timed_mode_same_trajectory(qpt, oracle, gpu):
     assert launches == 8 and ms > 0
     ref = oracle.solve(F, D, betas, 8, seed=4)
     np.testing.assert_array_equal(st["perm"], ref["final_perm"])
+
+
+def test_frozen_ladder_all_weights_zero(qpt, oracle, gpu):
+    """beta so large that 20/beta < 1: at a local minimum (max dE = 0) every weight
+    truncates to zero and the update must stay (the oracle's definition)."""
+    F, D = qpt.instances.random_instance(31, 12)
+    betas = np.array([5.0, 40.0, 400.0])
+    res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, 30, seed=5)
+    _assert_same(res, st, ver, ref)
+
+
+def test_large_asym_n200_lds_shapes(qpt, oracle, gpu):
+    F, D = qpt.instances.random_instance(32, 200, 0, 20, symmetric=False, zero_diag=False)
+    betas = qpt.instances.geometric_ladder(3, 50.0, 5000.0)
+    for shape in ("256,1", "128,2", "64,4"):
+        res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, 2, seed=6,
+                                      env={"QAP_PT_SHAPE": shape})
+        _assert_same(res, st, ver, ref)
