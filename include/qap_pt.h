@@ -125,6 +125,44 @@ int qap_pt_pool_state(qap_pt_pool *pool, int32_t inst, int32_t *perm, int64_t *E
 int qap_pt_pool_verify(qap_pt_pool *pool, int64_t *force_mismatches, int64_t *cost_mismatches);
 int qap_pt_pool_destroy(qap_pt_pool *pool);
 
+/* --- serving loop: continuous batching (no counterpart in the reference, which solves
+ *     one instance per process; this is the request loop around the same solve path).
+ *     Requests are copied at submit.  A worker thread admits queued requests in arrival
+ *     order into pools (up to max_batch per pool, max_pools in flight) whenever their
+ *     device footprint fits the budget; admitted pools run concurrently on their own
+ *     streams.  A request's result does not depend on what it was batched with. ------ */
+typedef struct qap_srv qap_srv;
+
+typedef struct {
+    int32_t max_pools;       /* pools in flight (>= 1), default 4                        */
+    int32_t max_batch;       /* requests merged into one pool (>= 1), default 64          */
+    int64_t device_budget;   /* device bytes the server may hold; 0 = free HBM - reserve  */
+    int64_t reserve_bytes;   /* kept free when device_budget == 0, default 1 GiB          */
+    double batch_window_s;   /* hold a partial batch up to this long for more requests    */
+} qap_srv_config;
+
+/* request status reported by qap_srv_poll / qap_srv_wait (negative = failed with that
+ * QAP_PT_E_* code, e.g. QAP_PT_E_NOMEM when it can never fit the budget) */
+#define QAP_SRV_QUEUED 0
+#define QAP_SRV_RUNNING 1
+#define QAP_SRV_DONE 2
+
+int qap_srv_create(const qap_srv_config *cfg, qap_srv **out); /* cfg NULL = defaults */
+/* Queues one request; QAP_PT_E_NOMEM at once if it alone exceeds an explicit budget. */
+int qap_srv_submit(qap_srv *srv, const qap_problem *prob, const qap_pt_params *par,
+                   uint64_t *ticket);
+/* Non-blocking.  On a final status (DONE or failed) res is filled -- res->best_perm is
+ * caller-allocated n or NULL; res->seconds is the submit-to-completion latency -- and
+ * the ticket is released. */
+int qap_srv_poll(qap_srv *srv, uint64_t ticket, qap_pt_result *res, int32_t *status);
+/* As poll, waiting up to timeout_s (<= 0: no limit) for a final status. */
+int qap_srv_wait(qap_srv *srv, uint64_t ticket, double timeout_s, qap_pt_result *res,
+                 int32_t *status);
+int qap_srv_stats(qap_srv *srv, int64_t *queued, int64_t *running, int64_t *done,
+                  int64_t *device_bytes);
+/* Finishes running pools, fails queued requests with QAP_PT_E_BUSY, frees everything. */
+int qap_srv_destroy(qap_srv *srv);
+
 /* Sweep-kernel shape chosen for an instance of size n ("threads x candidates per
  * thread, lds|hbm"), for reports. */
 int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen);

@@ -42,7 +42,17 @@ EXPORTS = (
     "qap_pt_pool_verify",
     "qap_pt_pool_destroy",
     "qap_pt_kernel_shape",
+    "qap_srv_create",
+    "qap_srv_submit",
+    "qap_srv_poll",
+    "qap_srv_wait",
+    "qap_srv_stats",
+    "qap_srv_destroy",
 )
+
+QAP_SRV_QUEUED = 0
+QAP_SRV_RUNNING = 1
+QAP_SRV_DONE = 2
 
 _I32P = C.POINTER(C.c_int32)
 _I64P = C.POINTER(C.c_int64)
@@ -85,6 +95,16 @@ class QapPtResult(C.Structure):
     ]
 
 
+class QapSrvConfig(C.Structure):
+    _fields_ = [
+        ("max_pools", C.c_int32),
+        ("max_batch", C.c_int32),
+        ("device_budget", C.c_int64),
+        ("reserve_bytes", C.c_int64),
+        ("batch_window_s", C.c_double),
+    ]
+
+
 class QapError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"qap_pt error {code}: {msg}")
@@ -119,6 +139,15 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.qap_pt_pool_verify.argtypes = [C.c_void_p, _I64P, _I64P]
     lib.qap_pt_pool_destroy.argtypes = [C.c_void_p]
     lib.qap_pt_kernel_shape.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
+    lib.qap_srv_create.argtypes = [C.POINTER(QapSrvConfig), C.POINTER(C.c_void_p)]
+    lib.qap_srv_submit.argtypes = [C.c_void_p, C.POINTER(QapProblem), C.POINTER(QapPtParams),
+                                   C.POINTER(C.c_uint64)]
+    lib.qap_srv_poll.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(QapPtResult),
+                                 C.POINTER(C.c_int32)]
+    lib.qap_srv_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(QapPtResult),
+                                 C.POINTER(C.c_int32)]
+    lib.qap_srv_stats.argtypes = [C.c_void_p, _I64P, _I64P, _I64P, _I64P]
+    lib.qap_srv_destroy.argtypes = [C.c_void_p]
     _lib = lib
     return lib
 
@@ -289,6 +318,77 @@ class Pool:
     def close(self):
         if self.handle:
             self.lib.qap_pt_pool_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop (qap_srv_*): continuous batching of solve requests on one GPU."""
+
+    def __init__(self, max_pools=4, max_batch=64, device_budget=0, reserve_bytes=1 << 30,
+                 batch_window_s=0.0):
+        self.lib = load_library()
+        cfg = QapSrvConfig(int(max_pools), int(max_batch), int(device_budget),
+                           int(reserve_bytes), float(batch_window_s))
+        self.handle = C.c_void_p()
+        _check(self.lib.qap_srv_create(C.byref(cfg), C.byref(self.handle)))
+        self._n = {}
+
+    def submit(self, inst: Instance) -> int:
+        prob, par = inst.c_structs()
+        t = C.c_uint64(0)
+        _check(self.lib.qap_srv_submit(self.handle, C.byref(prob), C.byref(par), C.byref(t)))
+        self._n[int(t.value)] = inst.n
+        return int(t.value)
+
+    def _final(self, t, res, perm, st):
+        self._n.pop(t, None)
+        out = _result_dict(res, perm)
+        out["status"] = int(st.value)
+        return out
+
+    def poll(self, t: int):
+        """None while queued/running; the result dict (with 'status') once final."""
+        perm = np.zeros(self._n[t], dtype=np.int32)
+        res = QapPtResult()
+        res.best_perm = _ptr(perm, C.c_int32)
+        st = C.c_int32(0)
+        _check(self.lib.qap_srv_poll(self.handle, t, C.byref(res), C.byref(st)))
+        if st.value in (QAP_SRV_QUEUED, QAP_SRV_RUNNING):
+            return None
+        return self._final(t, res, perm, st)
+
+    def wait(self, t: int, timeout_s: float = 0.0):
+        perm = np.zeros(self._n[t], dtype=np.int32)
+        res = QapPtResult()
+        res.best_perm = _ptr(perm, C.c_int32)
+        st = C.c_int32(0)
+        _check(self.lib.qap_srv_wait(self.handle, t, float(timeout_s), C.byref(res),
+                                     C.byref(st)))
+        if st.value in (QAP_SRV_QUEUED, QAP_SRV_RUNNING):
+            return None
+        return self._final(t, res, perm, st)
+
+    def stats(self) -> dict:
+        q, r, d, b = (C.c_int64(0) for _ in range(4))
+        _check(self.lib.qap_srv_stats(self.handle, C.byref(q), C.byref(r), C.byref(d),
+                                      C.byref(b)))
+        return {"queued": q.value, "running": r.value, "done": d.value, "device_bytes": b.value}
+
+    def close(self):
+        if self.handle:
+            self.lib.qap_srv_destroy(self.handle)
             self.handle = C.c_void_p()
 
     def __enter__(self):

@@ -81,6 +81,7 @@ struct InstDesc {
     int32_t n, ld;            // size, row stride of the force matrix in HBM (elements)
     int32_t ld_s;             // row stride of the LDS copy (elements, odd word stride)
     int32_t sparse_upd;       // compact the nonzero flow-difference columns on update
+    int32_t mul24;            // |u|, |v| of the rank-2 update fit the 24-bit multiplier
     int32_t R, T;             // replicas per ladder, ladders
     int32_t spe;              // sweeps per exchange
     int32_t kick_after;

@@ -24,10 +24,11 @@
 
 #include "../../include/qap_pt.h"
 #include "qap_pt_launch.h"
+#include "qap_pt_internal.h"
 
 using namespace qap;
 
-namespace {
+namespace qap {
 
 thread_local std::string g_last_error;
 
@@ -40,6 +41,10 @@ int fail(int code, const char *fmt, ...) {
     g_last_error = buf;
     return code;
 }
+
+}  // namespace qap
+
+namespace {
 
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
@@ -91,11 +96,21 @@ bool is_sym_zero_diag(int n, const int32_t *M) {
 
 int round4i(int x) { return (x + 3) & ~3; }
 
-// Odd word stride for the LDS copy of I: the column gather I[pinv[b]][a] then spreads
-// over all LDS banks.  For uint16 storage the stride in 32-bit words is ld_s / 2.
+// Row strides of the force matrix.  Every row has at least one padding column (index
+// n): lanes without a column of their own in the rank-2 update read and rewrite it, so
+// full row groups need no per-lane predication.  The int32 LDS copy has 16-byte rows
+// (an odd number of 4-word quads) so the update runs on 128-bit LDS accesses; the
+// uint16 copy uses an odd word stride (ld_s / 2 words) so the column gather
+// I[pinv[b]][a] spreads over all banks.
+int hbm_stride(int n) { return round4i(n + 1); }
+
 int lds_stride(int n, int im) {
-    if (im == IM_LDS32) return (n % 2) ? n : n + 1;
-    int ld = (n + 1) & ~1;
+    if (im == IM_LDS32) {  // 16-byte rows for 128-bit updates; odd number of 4-word quads
+        int q = (n + 1 + 3) / 4;
+        if (q % 2 == 0) ++q;
+        return 4 * q;
+    }
+    int ld = (n + 2) & ~1;
     if ((ld / 2) % 2 == 0) ld += 2;
     return ld;
 }
@@ -389,9 +404,11 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
         const SweepShape sh = choose_shape(n, sym, nonneg, bnd.force, &ld_s);
         p->shapes[k] = sh;
         d.n = n;
-        d.ld = round4i(n);
+        d.ld = hbm_stride(n);
         d.ld_s = ld_s;
         d.sparse_upd = offdiag_density(n, pb.F) < 0.35 ? 1 : 0;
+        // u = distance differences (<= 2 max|D|), v = flow differences (<= 4 max|F|)
+        d.mul24 = (2 * bnd.maxD < (1 << 23) && 4 * bnd.maxF < (1 << 23)) ? 1 : 0;
         d.R = pr.R;
         d.T = pr.T;
         d.spe = pr.sweeps_per_exchange;
@@ -777,9 +794,73 @@ int pool_result_impl(qap_pt_pool *p, int inst, qap_pt_result *res) {
 
 }  // namespace
 
+namespace qap {
+
+int pool_create_internal(int32_t count, const qap_problem *problems, const qap_pt_params *params,
+                         qap_pt_pool **out) {
+    try {
+        return pool_create_impl(count, problems, params, out);
+    } catch (const std::bad_alloc &) {
+        return fail(QAP_PT_E_NOMEM, "host allocation failed");
+    }
+}
+
+int pool_launch_async(qap_pt_pool *p, hipEvent_t done) {
+    HIP_TRY(hipSetDevice(p->device));
+    int64_t max_new = 0;
+    for (int k = 0; k < p->count; ++k) {
+        const qap_pt_params &pr = p->params[k];
+        const int64_t megas = pr.sweeps / pr.sweeps_per_exchange - p->limits[k];
+        if (megas > 0) {
+            p->limits[k] += megas;
+            max_new = std::max(max_new, megas);
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(p->d_limits, p->limits.data(), sizeof(int64_t) * p->count,
+                           hipMemcpyHostToDevice, p->stream));
+    HIP_TRY(launch_set_limits(p->d_state, p->d_limits, p->count, p->stream));
+    if (max_new > 0) {
+        const int depth = (int)std::max<int64_t>(1, std::min<int64_t>(max_new, 32));
+        if (!p->graph_exec || p->graph_depth != depth) {
+            const int rc = build_graph(p, depth);
+            if (rc) return rc;
+        }
+        for (int64_t m = 0; m < max_new; m += p->graph_depth)
+            HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
+    }
+    HIP_TRY(hipEventRecord(done, p->stream));
+    return QAP_PT_OK;
+}
+
+int pool_result_internal(qap_pt_pool *p, int inst, qap_pt_result *res) {
+    try {
+        return pool_result_impl(p, inst, res);
+    } catch (...) {
+        return fail(QAP_PT_E_NOMEM, "host exception in result");
+    }
+}
+
+void pool_destroy_internal(qap_pt_pool *p) { pool_free(p); }
+
+int64_t pool_device_bytes(int32_t count, const qap_problem *problems, const qap_pt_params *params) {
+    int64_t bytes = 0;
+    for (int k = 0; k < count; ++k) {
+        const int64_t n = problems[k].n, RT = (int64_t)params[k].R * params[k].T;
+        const int64_t ld = hbm_stride((int)n);
+        bytes += 4 * (4 * n * n + 3 * n);            // F, D, Ft, Dt, diagonals, classes
+        bytes += 4 * RT * n * ld;                     // force matrices
+        bytes += 4 * 3 * RT * n;                      // perm, pinv, best_perm
+        bytes += RT * (8 * 4 + 4 * 3);                // E, bestE, accepted, cells, pos, order, rep_inst
+        bytes += 24 * params[k].T + 16 * params[k].R + sizeof(InstDesc) + sizeof(InstState) + 8;
+    }
+    return bytes;
+}
+
+}  // namespace qap
+
 extern "C" {
 
-const char *qap_pt_last_error(void) { return g_last_error.c_str(); }
+const char *qap_pt_last_error(void) { return qap::g_last_error.c_str(); }
 
 int qap_pt_device_count(void) {
     int n = 0;

@@ -65,16 +65,141 @@ __device__ __forceinline__ int lane63(int v) { return __builtin_amdgcn_readlane(
 
 __host__ __device__ constexpr int round4(int x) { return (x + 3) & ~3; }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() also waits for every
+// outstanding global load (vmcnt(0)), which would stall the row prefetches; the sweep's
+// barriers only publish LDS data, so they fence the LDS address space alone.  With a
+// single wave no s_barrier is needed; the fence keeps the compiler's order.
+template <int NW>
+__device__ __forceinline__ void lds_sync() {
+    if constexpr (NW == 1) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront", "local");
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
+}
+
 // red_s (ints): [0,16) max partials, [16,80) sum partials [q*16 + wave],
 // [96] winner b, [97] winner delta, [98] winner's partner location j.
 constexpr int kRedWords = 128;
-constexpr int kSmallArrays = 15;  // p, pinv, cls, u1, u2, col, v1, v2, rnd, drow x2, dcol x2, ddiag, spare
+// p, pinv, cls, u1, u2, col, v1, v2, rnd, ddiag, drow x2, dcol x2, frow x2, ftrow x2
+constexpr int kSmallArrays = 18;
 
 __host__ __device__ inline int sweep_lds_words(int n, int ld_s, int im) {
     const int small = kSmallArrays * round4(n) + kRedWords;
     if (im == IM_LDS32) return small + n * ld_s;
     if (im == IM_LDS16) return small + round4((n * ld_s + 1) / 2);
     return small;
+}
+
+// Product of a distance difference and a flow difference in the rank-2 update.  With
+// |u| < 2^23 and |v| < 2^23 (checked on the host) and |u v| < 2^31 (the delta bound),
+// the 24-bit multiplier gives the exact product at full rate.
+template <bool M24>
+__device__ __forceinline__ int32_t umul(int32_t u, int32_t v) {
+    if constexpr (M24) return __mul24(u, v);
+    else return u * v;
+}
+
+// Dense rank-2 update I[k][c] += u1[k] v1[c] (+ u2[k] v2[c]) over the columns a thread
+// owns (colp[q] = column base, lanes without a column point at the padding column n,
+// where v = 0).  Rows go in groups of G with the next group's force entries and row
+// factors (four rows per 16-byte LDS read) in flight while the current group is written.
+template <int CPL, int IM, bool SYM, bool M24, typename IT>
+__device__ __forceinline__ void dense_update(IT *const colp[CPL], int32_t rs, int n,
+                                             const int32_t *u1_s, const int32_t *u2_s,
+                                             const int32_t v1[CPL], const int32_t v2[CPL]) {
+    constexpr int G = (CPL == 1) ? 16 : (CPL == 2 ? 8 : 4);
+    int32_t xc[G][CPL], xn[G][CPL], uc1[G], un1[G], uc2[G], un2[G];
+    auto load_group = [&](int k0) {
+#pragma unroll
+        for (int r4 = 0; r4 < G; r4 += 4) {
+            const int4 a = *reinterpret_cast<const int4 *>(u1_s + k0 + r4);
+            un1[r4] = a.x; un1[r4 + 1] = a.y; un1[r4 + 2] = a.z; un1[r4 + 3] = a.w;
+            if constexpr (!SYM) {
+                const int4 b = *reinterpret_cast<const int4 *>(u2_s + k0 + r4);
+                un2[r4] = b.x; un2[r4 + 1] = b.y; un2[r4 + 2] = b.z; un2[r4 + 3] = b.w;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            const int k = min(k0 + r, n - 1);
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) xn[r][q] = (int32_t)colp[q][(int64_t)k * rs];
+        }
+    };
+    load_group(0);
+    for (int k0 = 0; k0 < n; k0 += G) {
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            uc1[r] = un1[r];
+            if constexpr (!SYM) uc2[r] = un2[r];
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) xc[r][q] = xn[r][q];
+        }
+        if (k0 + G < n) load_group(k0 + G);
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            const int k = k0 + r;
+            if (k < n) {  // uniform
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const int32_t dv = SYM ? umul<M24>(uc1[r], v1[q])
+                                           : umul<M24>(uc1[r], v1[q]) + umul<M24>(uc2[r], v2[q]);
+                    if constexpr (IM == IM_LDS16) colp[q][(int64_t)k * rs] = (uint16_t)(xc[r][q] - dv);
+                    else colp[q][(int64_t)k * rs] = xc[r][q] + dv;
+                }
+            }
+        }
+    }
+}
+
+// Dense rank-2 update of an int32 LDS force matrix with 16-byte rows (ld4 quads per
+// row): each lane owns one 4-column quad and a fixed residue of rows, reads and writes
+// it with 128-bit LDS accesses (a single wave issues one LDS instruction per ~23 cycles,
+// so 4 columns per access is what bounds this loop), four rows in flight.
+template <int NT, bool SYM, bool M24>
+__device__ __forceinline__ void quad_update(int4 *I4, int ld4, int n, const int32_t *u1_s,
+                                            const int32_t *u2_s, const int32_t *v1_s,
+                                            const int32_t *v2_s, int tid) {
+    const int Qn = (n + 3) >> 2;                  // quads holding real columns
+    const int rpp = NT / Qn;                      // rows per pass (>= 1: n <= 4 NT)
+    const int cx = tid % Qn, ry = tid / Qn;
+    if (ry >= rpp) return;
+    const int4 a1 = *reinterpret_cast<const int4 *>(v1_s + 4 * cx);
+    int4 a2 = int4{0, 0, 0, 0};
+    if constexpr (!SYM) a2 = *reinterpret_cast<const int4 *>(v2_s + 4 * cx);
+    constexpr int G = 4;
+    for (int k0 = ry; k0 < n; k0 += G * rpp) {
+        int4 x[G];
+        int32_t w1[G], w2[G];
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            const int k = min(k0 + r * rpp, n - 1);
+            x[r] = I4[k * ld4 + cx];
+            w1[r] = u1_s[k];
+            if constexpr (!SYM) w2[r] = u2_s[k];
+        }
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            const int k = k0 + r * rpp;
+            if (k < n) {
+                int4 y = x[r];
+                y.x += umul<M24>(w1[r], a1.x);
+                y.y += umul<M24>(w1[r], a1.y);
+                y.z += umul<M24>(w1[r], a1.z);
+                y.w += umul<M24>(w1[r], a1.w);
+                if constexpr (!SYM) {
+                    y.x += umul<M24>(w2[r], a2.x);
+                    y.y += umul<M24>(w2[r], a2.y);
+                    y.z += umul<M24>(w2[r], a2.z);
+                    y.w += umul<M24>(w2[r], a2.w);
+                }
+                I4[k * ld4 + cx] = y;
+            }
+        }
+    }
 }
 
 template <int IM>
@@ -112,7 +237,9 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     int32_t *drow_s = reinterpret_cast<int32_t *>(rnd_s + n4);  // 2 buffers
     int32_t *dcol_s = drow_s + 2 * n4;                           // 2 buffers
     int32_t *ddiag_s = dcol_s + 2 * n4;
-    int32_t *red_s = ddiag_s + 2 * n4;  // one spare array keeps the 16-byte alignment
+    int32_t *frow_s = ddiag_s + n4;                              // 2 buffers
+    int32_t *ftrow_s = frow_s + 2 * n4;                          // 2 buffers
+    int32_t *red_s = ftrow_s + 2 * n4;
     IT *I_s = reinterpret_cast<IT *>(red_s + kRedWords);
 
     const int32_t *__restrict__ F = P.F + d.mat_off;
@@ -143,24 +270,71 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         cls_s[x] = P.fclass[d.vec_off + x];
         if constexpr (!SYM) ddiag_s[x] = Ddg[x];
     }
+    // HBM rows are 16-byte aligned (ld % 4 == 0): a (row, 4-column chunk) tiling of the
+    // workgroup with 8 chunk loads in flight per thread stages I into LDS.
+    const int L4 = ld >> 2;
     int cw_st = 1;
-    while (cw_st < n && cw_st < NT) cw_st <<= 1;
+    while (cw_st < L4 && cw_st < NT) cw_st <<= 1;
     const int sh_st = __builtin_ctz(cw_st);
-    if constexpr (IM != IM_HBM) {
-        const int rw = NT >> sh_st;
-        for (int k = tid >> sh_st; k < n; k += rw)
-            for (int c = tid & (cw_st - 1); c < n; c += cw_st) {
-                const int32_t v = Ig[(int64_t)k * ld + c];
-                if constexpr (IM == IM_LDS16) I_s[k * ld_s + c] = (uint16_t)(-v);
-                else I_s[k * ld_s + c] = v;
+    const int rw_st = NT >> sh_st;
+    const int cx_st = tid & (cw_st - 1), ry_st = tid >> sh_st;
+    constexpr int kStageU = 8;
+    if constexpr (IM == IM_LDS32) {  // 16-byte rows on both sides: straight quad copies
+        const int4 *Ig4 = reinterpret_cast<const int4 *>(Ig);
+        int4 *Is4 = reinterpret_cast<int4 *>(I_s);
+        const int ld4 = ld_s >> 2;
+        for (int cx = cx_st; cx < L4; cx += cw_st) {
+            for (int k0 = ry_st; k0 < n; k0 += rw_st * kStageU) {
+                int4 v[kStageU];
+#pragma unroll
+                for (int u = 0; u < kStageU; ++u) v[u] = Ig4[(int64_t)min(k0 + u * rw_st, n - 1) * L4 + cx];
+#pragma unroll
+                for (int u = 0; u < kStageU; ++u) {
+                    const int k = k0 + u * rw_st;
+                    if (k < n) Is4[k * ld4 + cx] = v[u];
+                }
             }
+        }
+    } else if constexpr (IM == IM_LDS16) {
+        const int4 *Ig4 = reinterpret_cast<const int4 *>(Ig);
+        for (int cx = cx_st; cx < L4; cx += cw_st) {
+            for (int k0 = ry_st; k0 < n; k0 += rw_st * kStageU) {
+                int4 v[kStageU];
+#pragma unroll
+                for (int u = 0; u < kStageU; ++u) {
+                    const int k = min(k0 + u * rw_st, n - 1);
+                    v[u] = Ig4[(int64_t)k * L4 + cx];
+                }
+#pragma unroll
+                for (int u = 0; u < kStageU; ++u) {
+                    const int k = k0 + u * rw_st;
+                    if (k < n) {
+                        const int c = 4 * cx;
+                        const int32_t e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            if (c + t < n) {
+                                if constexpr (IM == IM_LDS16) I_s[k * ld_s + c + t] = (uint16_t)(-e[t]);
+                                else I_s[k * ld_s + c + t] = e[t];
+                            }
+                        }
+                    }
+                }
+            }
+        }
     }
-    // distance row 0 (and column 0) into buffer 0
+    // distance row 0 (and column 0) and the flow row of p(0) into buffer 0
+    lds_sync<NW>();
+    int a_cur = p_s[0];
     for (int x = tid; x < n; x += NT) {
         drow_s[x] = D[x];
-        if constexpr (!SYM) dcol_s[x] = Dt[x];
+        frow_s[x] = F[(int64_t)a_cur * n + x];
+        if constexpr (!SYM) {
+            dcol_s[x] = Dt[x];
+            ftrow_s[x] = Ft[(int64_t)a_cur * n + x];
+        }
     }
-    __syncthreads();
+    lds_sync<NW>();
 
     int64_t E = P.E[g];
     int64_t bestE = P.bestE[g];
@@ -172,60 +346,43 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     const uint32_t sweep0 = P.state[inst].sweep_base;
     const bool sparse_upd = d.sparse_upd != 0;
 
-    // Per-thread copies of the flow row of the current facility a at owned columns.
-    int32_t fr[CPL], ftr[CPL], fdg[CPL];
-    int a_cur = p_s[0];
-    int32_t faa = SYM ? 0 : Fdg[a_cur];
+    // F[b][b] of the owned candidates (constant for the launch) and their locations
+    // pinv[b] (kept in registers, updated on every swap)
+    int32_t fdg[CPL], pv[CPL];
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
         const int b = tid + q * NT;
-        fr[q] = (b < n) ? F[(int64_t)a_cur * n + b] : 0;
-        ftr[q] = (!SYM && b < n) ? Ft[(int64_t)a_cur * n + b] : 0;
         fdg[q] = (!SYM && b < n) ? Fdg[b] : 0;
+        pv[q] = (b < n) ? pinv_s[b] : 0;
     }
     int buf = 0;
 
     for (int s = 0; s < d.spe; ++s) {
         const uint32_t sweep = sweep0 + (uint32_t)s;
-        __syncthreads();
+        lds_sync<NW>();
         for (int x = tid; x < n; x += NT)
             rnd_s[x] = philox4x32_10(u32x4{(uint32_t)x, sweep, (uint32_t)local, 0u}, d.seed_lo,
                                      d.seed_hi)
                            .x;
-        __syncthreads();
+        lds_sync<NW>();
 
         for (int i = 0; i < n; ++i) {
             const int a = p_s[i];
             if (a != a_cur) {  // the previous swap moved the facility we prefetched
                 a_cur = a;
-                if constexpr (!SYM) faa = Fdg[a];
-#pragma unroll
-                for (int q = 0; q < CPL; ++q) {
-                    const int b = tid + q * NT;
-                    fr[q] = (b < n) ? F[(int64_t)a * n + b] : 0;
-                    if constexpr (!SYM) ftr[q] = (b < n) ? Ft[(int64_t)a * n + b] : 0;
+                for (int x = tid; x < n; x += NT) {
+                    frow_s[buf * n4 + x] = F[(int64_t)a * n + x];
+                    if constexpr (!SYM) ftrow_s[buf * n4 + x] = Ft[(int64_t)a * n + x];
                 }
+                lds_sync<NW>();
             }
-            // prefetch for the next update: distance row i+1, flow row p(i+1)
-            const int inext = (i + 1 == n) ? 0 : i + 1;
-            const int a_nx = p_s[inext];
-            int32_t dn[CPL], dtn[CPL], frn[CPL], ftrn[CPL];
-            int32_t faa_n = 0;
-            if constexpr (!SYM) faa_n = Fdg[a_nx];
-#pragma unroll
-            for (int q = 0; q < CPL; ++q) {
-                const int b = tid + q * NT;
-                const bool ok = b < n;
-                dn[q] = ok ? D[(int64_t)inext * n + b] : 0;
-                frn[q] = ok ? F[(int64_t)a_nx * n + b] : 0;
-                dtn[q] = (!SYM && ok) ? Dt[(int64_t)inext * n + b] : 0;
-                ftrn[q] = (!SYM && ok) ? Ft[(int64_t)a_nx * n + b] : 0;
-            }
-
             const int32_t *drow = drow_s + buf * n4;
             const int32_t *dcol = dcol_s + buf * n4;
+            const int32_t *frow = frow_s + buf * n4;    // F[a][.]
+            const int32_t *ftrow = ftrow_s + buf * n4;  // F[.][a]
             const int32_t Iia = Iget(i, a);
             const int32_t Dii = SYM ? 0 : ddiag_s[i];
+            const int32_t faa = SYM ? 0 : frow[a];
 
             // (1) O(1) deltas of every candidate (CPU_PIM.cpp:220-232)
             int32_t dE[CPL];
@@ -235,18 +392,34 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                 const int b = tid + q * NT;
                 dE[q] = INT32_MIN;
                 if (b < n) {
-                    const int j = pinv_s[b];
+                    const int j = pv[q];
                     const int32_t force = Iget(i, b) + Iget(j, a) - Iia - Iget(j, b);
                     int32_t v;
                     if constexpr (SYM) {
-                        v = force - 4 * drow[j] * fr[q];
+                        v = force - 4 * drow[j] * frow[b];
                     } else {
                         const int32_t dist = Dii + ddiag_s[j] - drow[j] - dcol[j];
-                        const int32_t flow = faa + fdg[q] - fr[q] - ftr[q];
+                        const int32_t flow = faa + fdg[q] - frow[b] - ftrow[b];
                         v = force - dist * flow;
                     }
                     dE[q] = v;
                     lmax = max(lmax, v);
+                }
+            }
+            // prefetch for the next update: distance row i+1 and flow row p(i+1).  The
+            // loads feed LDS stores at the end of this update only, so nothing in
+            // between waits for them.
+            const int inext = (i + 1 == n) ? 0 : i + 1;
+            const int a_nx = p_s[inext];
+            int32_t dn[CPL], dtn[CPL], frn[CPL], ftrn[CPL];
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const int bc = min(tid + q * NT, n - 1);
+                dn[q] = D[(int64_t)inext * n + bc];
+                frn[q] = F[(int64_t)a_nx * n + bc];
+                if constexpr (!SYM) {
+                    dtn[q] = Dt[(int64_t)inext * n + bc];
+                    ftrn[q] = Ft[(int64_t)a_nx * n + bc];
                 }
             }
             int maxd;
@@ -256,7 +429,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                     maxd = lane63(wm);
                 } else {
                     if (lane == kWave - 1) red_s[wid] = wm;
-                    __syncthreads();
+                    lds_sync<NW>();
                     maxd = red_s[0];
 #pragma unroll
                     for (int w = 1; w < NW; ++w) maxd = max(maxd, red_s[w]);
@@ -284,7 +457,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
 #pragma unroll
                     for (int q = 0; q < CPL; ++q) red_s[16 + q * 16 + wid] = (int)incl[q];
                 }
-                __syncthreads();
+                lds_sync<NW>();
 #pragma unroll
                 for (int q = 0; q < CPL; ++q) {
                     uint32_t tq = 0, before = 0;
@@ -303,45 +476,82 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
             // (3) winner: first b whose inclusive prefix exceeds floor(u * S / 2^32);
             //     it commits the permutation swap before the broadcast barrier.
             const uint32_t thr = (uint32_t)(((uint64_t)rnd_s[i] * (uint64_t)S) >> 32);
-            if (S == 0 && tid == 0) {  // every weight truncated to zero: stay
-                red_s[96] = a;
-                red_s[97] = 0;
-                red_s[98] = i;
-            }
+            int my_b = a, my_j = i;
+            int32_t my_d = 0;
+            bool owner = false;
 #pragma unroll
             for (int q = 0; q < CPL; ++q) {
                 const uint32_t ex = qbase[q] + wbase[q] + incl[q] - w[q];
                 if (ex <= thr && thr < ex + w[q]) {
-                    const int b = tid + q * NT;
-                    const int j = pinv_s[b];
-                    red_s[96] = b;
-                    red_s[97] = dE[q];
-                    red_s[98] = j;
-                    if (b != a) {
-                        p_s[i] = b;
-                        p_s[j] = a;
-                        pinv_s[b] = i;
-                        pinv_s[a] = j;
-                    }
+                    owner = true;
+                    my_b = tid + q * NT;
+                    my_j = pv[q];
+                    my_d = dE[q];
                 }
             }
-            // next distance row into the other buffer (nobody reads it this update)
+            if (owner && my_b != a) {  // the winner commits the permutation swap
+                p_s[i] = my_b;
+                p_s[my_j] = a;
+            }
+            // next rows into the other buffers (nobody reads them this update)
             {
                 int32_t *dnb = drow_s + (buf ^ 1) * n4;
                 int32_t *dtb = dcol_s + (buf ^ 1) * n4;
+                int32_t *fnb = frow_s + (buf ^ 1) * n4;
+                int32_t *ftb = ftrow_s + (buf ^ 1) * n4;
 #pragma unroll
                 for (int q = 0; q < CPL; ++q) {
-                    const int b = tid + q * NT;
-                    if (b < n) {
-                        dnb[b] = dn[q];
-                        if constexpr (!SYM) dtb[b] = dtn[q];
+                    const int bb = tid + q * NT;
+                    if (bb < n) {
+                        dnb[bb] = dn[q];
+                        fnb[bb] = frn[q];
+                        if constexpr (!SYM) {
+                            dtb[bb] = dtn[q];
+                            ftb[bb] = ftrn[q];
+                        }
                     }
                 }
             }
-            __syncthreads();
-            const int b = red_s[96];
-            const int32_t dsel = red_s[97];
-            const int j = red_s[98];
+            int b, j;
+            int32_t dsel;
+            if constexpr (NW == 1) {
+                // one wave: the winner's values travel by readlane, no LDS round trip
+                const uint64_t bal = __ballot(owner);
+                if (bal) {
+                    const int src = __ffsll((unsigned long long)bal) - 1;
+                    b = __builtin_amdgcn_readlane(my_b, src);
+                    j = __builtin_amdgcn_readlane(my_j, src);
+                    dsel = __builtin_amdgcn_readlane(my_d, src);
+                } else {  // every weight truncated to zero: stay
+                    b = a;
+                    j = i;
+                    dsel = 0;
+                }
+                lds_sync<NW>();
+            } else {
+                if (owner) {
+                    red_s[96] = my_b;
+                    red_s[97] = my_d;
+                    red_s[98] = my_j;
+                }
+                if (S == 0 && tid == 0) {  // every weight truncated to zero: stay
+                    red_s[96] = a;
+                    red_s[97] = 0;
+                    red_s[98] = i;
+                }
+                lds_sync<NW>();
+                b = red_s[96];
+                dsel = red_s[97];
+                j = red_s[98];
+            }
+            if (b != a) {  // owned inverse-permutation entries follow the swap
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const int bb = tid + q * NT;
+                    if (bb == b) pv[q] = i;
+                    else if (bb == a) pv[q] = j;
+                }
+            }
 
             if (b != a) {
                 // (4) commit (CPU_PIM.cpp:252-283)
@@ -355,17 +565,21 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                         const int c = tid + q * NT;
                         v1[q] = 0;
                         v2[q] = 0;
+                        const int cc = min(c, n - 1);
+                        const int32_t fb = F[(int64_t)b * n + cc];
+                        const int32_t dj = D[(int64_t)j * n + cc];
+                        int32_t ftb = 0, dtj = 0;
+                        if constexpr (!SYM) {
+                            ftb = Ft[(int64_t)b * n + cc];
+                            dtj = Dt[(int64_t)j * n + cc];
+                        }
                         if (c < n) {
-                            const int32_t fb = F[(int64_t)b * n + c];
-                            const int32_t dj = D[(int64_t)j * n + c];
                             if constexpr (SYM) {
-                                v1[q] = 2 * (fb - fr[q]);           // 2 (F[b][c] - F[a][c])
+                                v1[q] = 2 * (fb - frow[c]);          // 2 (F[b][c] - F[a][c])
                                 u1_s[c] = dj - drow[c];              // D[j][k] - D[i][k]
                             } else {
-                                const int32_t ftb = Ft[(int64_t)b * n + c];
-                                const int32_t dtj = Dt[(int64_t)j * n + c];
-                                v1[q] = ftb - ftr[q];                // F[c][b] - F[c][a]
-                                v2[q] = fb - fr[q];                  // F[b][c] - F[a][c]
+                                v1[q] = ftb - ftrow[c];              // F[c][b] - F[c][a]
+                                v2[q] = fb - frow[c];                // F[b][c] - F[a][c]
                                 u1_s[c] = dtj - dcol[c];             // D[k][j] - D[k][i]
                                 u2_s[c] = dj - drow[c];              // D[j][k] - D[i][k]
                             }
@@ -373,6 +587,16 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                         }
                     }
                     uint32_t nnz = (uint32_t)n;
+                    if (IM == IM_LDS32 && !sparse_upd) {  // quad update reads v from LDS
+#pragma unroll
+                        for (int q = 0; q < CPL; ++q) {
+                            const int c = tid + q * NT;
+                            if (c < n4) {
+                                v1_s[c] = v1[q];
+                                if constexpr (!SYM) v2_s[c] = v2[q];
+                            }
+                        }
+                    }
                     if (sparse_upd) {
                         const uint32_t inc = wave_incl_sum(nz);
                         uint32_t base;
@@ -381,7 +605,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                             nnz = (uint32_t)lane63((int)inc);
                         } else {
                             if (lane == kWave - 1) red_s[16 + wid] = (int)inc;
-                            __syncthreads();
+                            lds_sync<NW>();
                             uint32_t before = 0, tot = 0;
 #pragma unroll
                             for (int ww = 0; ww < NW; ++ww) {
@@ -402,40 +626,23 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                             }
                         }
                     }
-                    __syncthreads();
+                    lds_sync<NW>();
                     cells += (uint64_t)n * nnz;
-                    if (!sparse_upd) {
-                        // owned columns, four rows in flight
-                        for (int k0 = 0; k0 < n; k0 += 4) {
-                            int32_t x[4][CPL], uk1[4], uk2[4];
+                    if (IM == IM_LDS32 && !sparse_upd) {
+                        int4 *I4 = reinterpret_cast<int4 *>(I_s);
+                        if (d.mul24)
+                            quad_update<NT, SYM, true>(I4, ld_s >> 2, n, u1_s, u2_s, v1_s, v2_s, tid);
+                        else
+                            quad_update<NT, SYM, false>(I4, ld_s >> 2, n, u1_s, u2_s, v1_s, v2_s, tid);
+                    } else if (!sparse_upd) {
+                        IT *colp[CPL];
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const int k = k0 + r;
-                                uk1[r] = (k < n) ? u1_s[k] : 0;
-                                uk2[r] = (!SYM && k < n) ? u2_s[k] : 0;
-#pragma unroll
-                                for (int q = 0; q < CPL; ++q) {
-                                    const int c = tid + q * NT;
-                                    x[r][q] = (k < n && c < n) ? *Iaddr(k, c) : 0;
-                                }
-                            }
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const int k = k0 + r;
-#pragma unroll
-                                for (int q = 0; q < CPL; ++q) {
-                                    const int c = tid + q * NT;
-                                    if (k < n && c < n) {
-                                        const int32_t dv = SYM ? uk1[r] * v1[q]
-                                                               : uk1[r] * v1[q] + uk2[r] * v2[q];
-                                        if constexpr (IM == IM_LDS16)
-                                            *Iaddr(k, c) = (uint16_t)(x[r][q] - dv);
-                                        else
-                                            *Iaddr(k, c) = x[r][q] + dv;
-                                    }
-                                }
-                            }
-                        }
+                        for (int q = 0; q < CPL; ++q) colp[q] = Iaddr(0, min(tid + q * NT, n));
+                        const int32_t rs = (IM == IM_HBM) ? ld : ld_s;
+                        if (d.mul24)
+                            dense_update<CPL, IM, SYM, true>(colp, rs, n, u1_s, u2_s, v1, v2);
+                        else
+                            dense_update<CPL, IM, SYM, false>(colp, rs, n, u1_s, u2_s, v1, v2);
                     } else if (nnz > 0) {
                         int cw = 1;
                         while (cw < (int)nnz && cw < NT) cw <<= 1;
@@ -443,63 +650,82 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                         const int rw = NT >> shw;
                         const int cx = tid & (cw - 1);
                         const int ry = tid >> shw;
-                        for (int ci = cx; ci < (int)nnz; ci += cw) {
-                            const int c = col_s[ci];
-                            const int32_t a1 = v1_s[ci];
-                            const int32_t a2 = v2_s[ci];
-                            for (int k0 = ry; k0 < n; k0 += 4 * rw) {
-                                int32_t x[4];
+                        const int64_t rs = (IM == IM_HBM) ? (int64_t)ld : (int64_t)ld_s;
+                        constexpr int G = 8;
+                        // lanes past the compacted list work on the padding column
+                        for (int ci0 = 0; ci0 < (int)nnz; ci0 += cw) {
+                            const int ci = ci0 + cx;
+                            const bool live = ci < (int)nnz;
+                            const int c = live ? col_s[ci] : n;
+                            const int32_t a1 = live ? v1_s[ci] : 0;
+                            const int32_t a2 = live ? v2_s[ci] : 0;
+                            IT *cp = Iaddr(0, c);
+                            for (int k0 = ry; k0 < n; k0 += G * rw) {
+                                int32_t x[G], w1[G], w2[G];
 #pragma unroll
-                                for (int r = 0; r < 4; ++r) {
-                                    const int k = k0 + r * rw;
-                                    x[r] = (k < n) ? *Iaddr(k, c) : 0;
+                                for (int r = 0; r < G; ++r) {
+                                    const int k = min(k0 + r * rw, n - 1);
+                                    x[r] = (int32_t)cp[k * rs];
+                                    w1[r] = u1_s[k];
+                                    if constexpr (!SYM) w2[r] = u2_s[k];
                                 }
 #pragma unroll
-                                for (int r = 0; r < 4; ++r) {
+                                for (int r = 0; r < G; ++r) {
                                     const int k = k0 + r * rw;
+                                    const int32_t dv = SYM ? w1[r] * a1 : w1[r] * a1 + w2[r] * a2;
                                     if (k < n) {
-                                        const int32_t dv = SYM ? u1_s[k] * a1
-                                                               : u1_s[k] * a1 + u2_s[k] * a2;
                                         if constexpr (IM == IM_LDS16)
-                                            *Iaddr(k, c) = (uint16_t)(x[r] - dv);
+                                            cp[k * rs] = (uint16_t)(x[r] - dv);
                                         else
-                                            *Iaddr(k, c) = x[r] + dv;
+                                            cp[k * rs] = x[r] + dv;
                                     }
                                 }
                             }
                         }
                     }
-                    __syncthreads();
+                    if constexpr (IM == IM_HBM) __syncthreads();
+                    else lds_sync<NW>();
                 }
                 if (E < bestE) {  // CPU_PIM.cpp:271-272, plus the permutation
                     bestE = E;
                     for (int x = tid; x < n; x += NT) bestpg[x] = p_s[x];
                 }
             }
-            // rotate the prefetched rows in
+            // the prefetched rows become current
             buf ^= 1;
             a_cur = a_nx;
-            if constexpr (!SYM) faa = faa_n;
-#pragma unroll
-            for (int q = 0; q < CPL; ++q) {
-                fr[q] = frn[q];
-                if constexpr (!SYM) ftr[q] = ftrn[q];
-            }
         }
     }
 
-    __syncthreads();
-    for (int x = tid; x < n; x += NT) {
-        pg[x] = p_s[x];
-        pinvg[x] = pinv_s[x];
+    lds_sync<NW>();
+    for (int x = tid; x < n; x += NT) pg[x] = p_s[x];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+        const int bb = tid + q * NT;
+        if (bb < n) pinvg[bb] = pv[q];
     }
-    if constexpr (IM != IM_HBM) {
-        const int rw = NT >> sh_st;
-        for (int k = tid >> sh_st; k < n; k += rw)
-            for (int c = tid & (cw_st - 1); c < n; c += cw_st) {
-                if constexpr (IM == IM_LDS16) Ig[(int64_t)k * ld + c] = -(int32_t)I_s[k * ld_s + c];
-                else Ig[(int64_t)k * ld + c] = I_s[k * ld_s + c];
+    if constexpr (IM == IM_LDS32) {
+        int4 *Ig4 = reinterpret_cast<int4 *>(Ig);
+        const int4 *Is4 = reinterpret_cast<const int4 *>(I_s);
+        const int ld4 = ld_s >> 2;
+        for (int cx = cx_st; cx < L4; cx += cw_st)
+            for (int k = ry_st; k < n; k += rw_st) Ig4[(int64_t)k * L4 + cx] = Is4[k * ld4 + cx];
+    } else if constexpr (IM == IM_LDS16) {
+        int4 *Ig4 = reinterpret_cast<int4 *>(Ig);
+        for (int cx = cx_st; cx < L4; cx += cw_st) {
+            for (int k = ry_st; k < n; k += rw_st) {
+                const int c = 4 * cx;
+                int32_t e[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int cc = min(c + t, n - 1);
+                    const int32_t v = (IM == IM_LDS16) ? -(int32_t)I_s[k * ld_s + cc]
+                                                       : (int32_t)I_s[k * ld_s + cc];
+                    e[t] = (c + t < n) ? v : 0;
+                }
+                Ig4[(int64_t)k * L4 + cx] = int4{e[0], e[1], e[2], e[3]};
             }
+        }
     }
     if (tid == 0) {
         P.E[g] = E;

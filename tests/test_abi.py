@@ -61,3 +61,30 @@ def test_no_device_is_an_error_not_a_fallback(qpt):
         pytest.skip("a HIP device is visible")
     F, D = qpt.instances.random_instance(2, 8)
     _expect(qpt, qpt.QAP_PT_E_DEVICE, qpt.Instance(F=F, D=D, beta_ln=np.array([0.1]), sweeps=2))
+
+
+def test_server_refuses_request_larger_than_budget(qpt):
+    """Admission is gated on device memory: with an explicit budget a request whose
+    footprint alone exceeds it is refused at submit (host side, no device needed)."""
+    F, D = qpt.instances.random_instance(3, 64)
+    inst = qpt.Instance(F=F, D=D, beta_ln=np.linspace(0.1, 1.0, 16), sweeps=4)
+    with qpt.Server(device_budget=100_000) as srv:
+        with pytest.raises(qpt.QapError) as ei:
+            srv.submit(inst)
+        assert ei.value.code == qpt.QAP_PT_E_NOMEM
+        small = qpt.Instance(F=F[:8, :8].copy(), D=D[:8, :8].copy(), beta_ln=np.array([0.1]),
+                             sweeps=2)
+        t = srv.submit(small)
+        r = srv.wait(t, timeout_s=30.0)
+        if qpt.device_count() == 0:
+            assert r is not None and r["status"] == qpt.QAP_PT_E_DEVICE
+        else:
+            assert r is not None and r["status"] == qpt.QAP_SRV_DONE
+
+
+def test_server_shutdown_fails_queued_requests(qpt):
+    F, D = qpt.instances.random_instance(4, 10)
+    srv = qpt.Server(max_pools=1)
+    for _ in range(3):
+        srv.submit(qpt.Instance(F=F, D=D, beta_ln=np.array([0.1, 0.2]), sweeps=2))
+    srv.close()  # must return: queued requests are failed, running pools finish
