@@ -117,7 +117,29 @@ def main():
     ap.add_argument("--sweeps-per-step", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one device-to-device copy of 256 MiB (calibrates the HBM "
+                         "traffic counters against a kernel of known size) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import ctypes
+
+        hip = ctypes.CDLL("libamdhip64.so")
+        nbytes = 256 << 20
+        src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+        for ptr in (src, dst):
+            if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes)) != 0:
+                raise SystemExit("hipMalloc failed")
+        hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+        hip.hipDeviceSynchronize()
+        # hipMemcpyDeviceToDevice = 3
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) != 0:
+            raise SystemExit("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        print(json.dumps({"copy_probe_bytes": nbytes}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every instance and ladder "
                                    "is independent, multi-GPU sharding is described in "
@@ -130,6 +152,7 @@ def main():
                                    "large512": 10, "large1024": 4}[args.workload]
 
     pool = pkg.Pool(insts)
+    shape = pool.shape(0)
     if args.warmup:
         pool.run(args.warmup * sps)
     before = [pool.result(k) for k in range(len(insts))]
@@ -151,7 +174,7 @@ def main():
     achieved = alg / max(launches, 1) / avg_launch_s / 1e9
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.txt")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
@@ -169,7 +192,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": dict(desc, sweeps_per_step=sps, kernel_shape=pkg.kernel_shape(insts[0].n)),
+        "config": dict(desc, sweeps_per_step=sps, kernel_shape=shape),
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,

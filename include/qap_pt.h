@@ -58,8 +58,8 @@ typedef struct {
 typedef struct {
     int32_t R;                   /* replicas per ladder (>= 1)                            */
     int32_t T;                   /* independent ladders (>= 1)                            */
-    const double *beta_ln;       /* R inverse temperatures in natural-log units, strictly
-                                    positive and ascending (index 0 = hottest); converted
+    const double *beta_ln;       /* R inverse temperatures in natural-log units, > 0 and
+                                    non-decreasing (index 0 = hottest); converted
                                     to exp2 units with the reference's 1.44269504089      */
     int64_t sweeps;              /* sweeps per replica, a multiple of sweeps_per_exchange */
     int32_t sweeps_per_exchange; /* NUMITERA (CPU_PIM.cpp:19)                             */
@@ -163,9 +163,12 @@ int qap_srv_stats(qap_srv *srv, int64_t *queued, int64_t *running, int64_t *done
 /* Finishes running pools, fails queued requests with QAP_PT_E_BUSY, frees everything. */
 int qap_srv_destroy(qap_srv *srv);
 
-/* Sweep-kernel shape chosen for an instance of size n ("threads x candidates per
- * thread, lds|hbm"), for reports. */
+/* Sweep-kernel shape chosen for a symmetric instance of size n with a large force
+ * bound ("threads x candidates per thread, lds32|lds16|hbm"), for reports. */
 int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen);
+/* Shape actually used for instance inst of a pool ("NTxCPL,placement,sym|general,
+ * dense-flow|sparse-flow"). */
+int qap_pt_pool_shape(qap_pt_pool *pool, int32_t inst, char *buf, int32_t buflen);
 
 #ifdef __cplusplus
 }

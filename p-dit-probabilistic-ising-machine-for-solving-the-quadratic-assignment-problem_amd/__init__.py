@@ -42,6 +42,7 @@ EXPORTS = (
     "qap_pt_pool_verify",
     "qap_pt_pool_destroy",
     "qap_pt_kernel_shape",
+    "qap_pt_pool_shape",
     "qap_srv_create",
     "qap_srv_submit",
     "qap_srv_poll",
@@ -139,6 +140,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.qap_pt_pool_verify.argtypes = [C.c_void_p, _I64P, _I64P]
     lib.qap_pt_pool_destroy.argtypes = [C.c_void_p]
     lib.qap_pt_kernel_shape.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
+    lib.qap_pt_pool_shape.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]
     lib.qap_srv_create.argtypes = [C.POINTER(QapSrvConfig), C.POINTER(C.c_void_p)]
     lib.qap_srv_submit.argtypes = [C.c_void_p, C.POINTER(QapProblem), C.POINTER(QapPtParams),
                                    C.POINTER(C.c_uint64)]
@@ -308,6 +310,11 @@ class Pool:
                                           _ptr(bperm, C.c_int32)))
         return {"perm": perm, "E": E, "bestE": bE, "pos": pos, "accepted": acc,
                 "best_perms": bperm}
+
+    def shape(self, k: int = 0) -> str:
+        buf = C.create_string_buffer(96)
+        _check(self.lib.qap_pt_pool_shape(self.handle, k, buf, 96))
+        return buf.value.decode()
 
     def verify(self) -> tuple[int, int]:
         fm = C.c_int64(0)

@@ -1,0 +1,283 @@
+// qap_pt_cli.cpp -- command-line front end with the reference solver's flags and log
+// lines (CPU-PIM/CPU_PIM.cpp:478-685), on top of the C-ABI in include/qap_pt.h.
+//
+//   qap_pt_cli [flags] FILE
+//     -b beta      largest beta in natural-log units (CPU_PIM.cpp:526-533); without -b the
+//                  ladder is chosen automatically (CPU_PIM.cpp:366-476, see below)
+//     -m ratio     multiplicative beta step between rungs (:555-561)
+//     -r R         replicas per ladder (:569-580), default 10
+//     -t T         independent ladders (the reference's threads, :588-599), default 20
+//     -w megas     exchange intervals (:600-606), default 10000
+//     -i k         sweeps per exchange interval (NUMITERA, :548-554), default 4
+//     -g goal      stop when the best cost is <= goal (:534-540)
+//     -G goal      print the best permutation if the best cost is < goal (:541-547)
+//     -s seconds   timeout (:581-587)
+//     -o dir       output folder for log.txt / sweep<t>.txt (:562-568)
+//     -al / -ah    automatic-mode proportions (:508-524)
+//     --seed S     Philox seed (the reference seeds from std::random_device)
+//     --tmin T --tmax T   geometric temperature ladder instead of -b/-m
+//
+// FILE holds n, then F (n*n), then D (n*n), whitespace separated (readProblem,
+// CPU_PIM.cpp:79-107).  Progress lines are written after every chunk of exchange
+// intervals, so the "Time (s)" of an improvement is resolved to one chunk.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "../../include/qap_pt.h"
+
+namespace {
+
+struct Options {
+    std::string file, outdir;
+    bool have_beta = false;
+    double beta0_ln = 0.0, ratio = 1.2;
+    int R = 10, T = 20, megas = 10000, numitera = 4;
+    bool have_goal = false;
+    long long goal = 0, goal_print = 0;
+    double timeout = 0.0;
+    double lower_prop = 0.1, higher_prop = 0.9;
+    unsigned long long seed = 0x5eedULL;
+    double tmin = 0.0, tmax = 0.0;
+};
+
+bool read_problem(const std::string &path, int &n, std::vector<int32_t> &F, std::vector<int32_t> &D) {
+    std::ifstream in(path);
+    if (!in.is_open()) {
+        std::cerr << "Error: Could not open the file (" << path << ")." << std::endl;
+        return false;
+    }
+    in >> n;
+    if (!in || n < 1 || n > QAP_PT_MAX_N) {
+        std::cerr << "Error: N must be in [1, " << QAP_PT_MAX_N << "]." << std::endl;
+        return false;
+    }
+    F.resize((size_t)n * n);
+    D.resize((size_t)n * n);
+    for (auto &x : F) in >> x;
+    for (auto &x : D) in >> x;
+    if (!in) {
+        std::cerr << "Error: the file ends before 2 n^2 matrix entries." << std::endl;
+        return false;
+    }
+    return true;
+}
+
+int check(int rc) {
+    if (rc != QAP_PT_OK) std::cerr << "qap_pt error " << rc << ": " << qap_pt_last_error() << std::endl;
+    return rc;
+}
+
+// Automatic ladder (CPU_PIM.cpp:366-476): for each exponent b in [-30, 1] run 5 trials of
+// T ladders x R replicas, all at beta = 2^b (exp2 units), for 10 exchange intervals from
+// fresh shuffles, and average the final cost of every replica.  The whole sweep of
+// 32 x 5 trials is one batched pool.  The selection rule is the reference's.
+bool auto_select(const Options &o, int n, const std::vector<int32_t> &F, const std::vector<int32_t> &D,
+                 double &beta0_ln, double &ratio, std::ostream &log) {
+    const double kMin = -30, kMax = 1, kStep = 1;
+    const int steps = (int)((kMax - kMin) / kStep) + 1, trials = 5, megas = 10;
+    std::vector<qap_problem> probs;
+    std::vector<qap_pt_params> pars;
+    std::vector<std::vector<double>> ladders;
+    for (int i = 0; i < steps; ++i) {
+        const double b2 = std::exp2(kMin + i * kStep);
+        ladders.emplace_back(o.R, b2 / 1.44269504089);  // beta_ln with the reference constant
+    }
+    for (int i = 0; i < steps; ++i)
+        for (int t = 0; t < trials; ++t) {
+            probs.push_back(qap_problem{n, F.data(), D.data()});
+            qap_pt_params p{};
+            p.R = o.R;
+            p.T = o.T;
+            p.beta_ln = ladders[i].data();
+            p.sweeps = (int64_t)megas * o.numitera;
+            p.sweeps_per_exchange = o.numitera;
+            p.seed = o.seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(1 + i * trials + t));
+            p.kick_after = 100;
+            pars.push_back(p);
+        }
+    qap_pt_pool *pool = nullptr;
+    if (check(qap_pt_pool_create((int32_t)probs.size(), probs.data(), pars.data(), &pool))) return false;
+    if (check(qap_pt_pool_run(pool, -1, nullptr, nullptr))) return false;
+    std::vector<double> avg(steps, 0.0);
+    std::vector<int64_t> E((size_t)o.R * o.T);
+    for (int i = 0; i < steps; ++i)
+        for (int t = 0; t < trials; ++t) {
+            if (check(qap_pt_pool_state(pool, i * trials + t, nullptr, E.data(), nullptr, nullptr,
+                                        nullptr, nullptr)))
+                return false;
+            for (auto e : E) avg[i] += (double)e / ((double)trials * o.T * o.R);
+        }
+    qap_pt_pool_destroy(pool);
+    for (int i = 0; i < steps; ++i) {
+        log << "Beta: " << std::exp2(kMin + i * kStep) << ", Average Energy: " << avg[i] << std::endl;
+        std::cout << "Beta: " << std::exp2(kMin + i * kStep) << ", Average Energy: " << avg[i] << std::endl;
+    }
+    // CPU_PIM.cpp:434-473 (evaluated once the last exponent has run)
+    const int last = steps - 1;
+    double minE = 1e30, maxE = -1e30, minBeta = 0, maxBeta = 0;
+    int minIdx = 0;
+    for (int j = 0; j < last; ++j) {
+        if (avg[j] < minE) {
+            minE = avg[j];
+            minIdx = j;
+        }
+        if (j == 0) maxE = avg[j];
+    }
+    for (int j = 0; j < last; ++j) {
+        if (minBeta == 0 && ((j == minIdx - 1 && j >= 1) || avg[j] < (maxE - minE) * o.higher_prop + minE))
+            minBeta = std::exp2(kMin + j * kStep);
+        if (j == last - 1 ||
+            (minBeta != 0 && (j == minIdx || avg[j] < (maxE - minE) * o.lower_prop + minE))) {
+            if (minBeta == 0) minBeta = std::exp2(kMin);
+            maxBeta = std::exp2(kMin + (j + 2) * kStep);
+            break;
+        }
+    }
+    log << "Selecting minimum beta of " << minBeta << std::endl << "Selecting maximum beta of " << maxBeta << std::endl;
+    std::cout << "Selecting minimum beta of " << minBeta << std::endl
+              << "Selecting maximum beta of " << maxBeta << std::endl;
+    beta0_ln = maxBeta / 1.44269504089;
+    ratio = std::pow(maxBeta / minBeta, 1.0 / o.R);
+    return true;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    Options o;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        auto next = [&](const char *flag) -> const char * {
+            if (i + 1 >= argc) {
+                std::cerr << "Error: " << flag << " option requires a value." << std::endl;
+                std::exit(1);
+            }
+            return argv[++i];
+        };
+        if (a == "-h" || a == "--help") {
+            std::cout << "Usage: " << argv[0] << " [OPTIONS] FILE\n"
+                      << "  -b -m -r -t -w -i -g -G -s -o -al -ah as the reference CPU_PIM;\n"
+                      << "  --seed S, --tmin T --tmax T (geometric temperature ladder)\n";
+            return 0;
+        } else if (a == "-b") {
+            o.beta0_ln = std::atof(next("-b"));
+            o.have_beta = true;
+        } else if (a == "-m") o.ratio = std::atof(next("-m"));
+        else if (a == "-r") o.R = std::atoi(next("-r"));
+        else if (a == "-t") o.T = std::atoi(next("-t"));
+        else if (a == "-w") o.megas = std::atoi(next("-w"));
+        else if (a == "-i") o.numitera = std::atoi(next("-i"));
+        else if (a == "-g") {
+            o.goal = std::atoll(next("-g"));
+            o.have_goal = true;
+        } else if (a == "-G") o.goal_print = std::atoll(next("-G"));
+        else if (a == "-s") o.timeout = std::atof(next("-s"));
+        else if (a == "-o") o.outdir = next("-o");
+        else if (a == "-al") o.lower_prop = std::atof(next("-al"));
+        else if (a == "-ah") o.higher_prop = std::atof(next("-ah"));
+        else if (a == "--seed") o.seed = std::strtoull(next("--seed"), nullptr, 0);
+        else if (a == "--tmin") o.tmin = std::atof(next("--tmin"));
+        else if (a == "--tmax") o.tmax = std::atof(next("--tmax"));
+        else o.file = a;
+    }
+    if (o.file.empty()) {
+        std::cerr << "Error: No input file specified." << std::endl;
+        return 1;
+    }
+    if (o.R < 1 || o.T < 1 || o.megas < 0 || o.numitera < 1) {
+        std::cerr << "Error: -r, -t, -i must be >= 1 and -w >= 0." << std::endl;
+        return 1;
+    }
+    if (o.outdir.empty()) o.outdir = o.file + ".out";
+    if (mkdir(o.outdir.c_str(), 0755) != 0)
+        std::cerr << "Error: Folder '" << o.outdir << "' already exists or could not be created." << std::endl;
+    int n = 0;
+    std::vector<int32_t> F, D;
+    if (!read_problem(o.file, n, F, D)) return 1;
+
+    std::ofstream log(o.outdir + "/log.txt");
+    const auto t0 = std::chrono::steady_clock::now();
+    log << "Iterations between swaps " << o.numitera << std::endl
+        << "Number of Swaps " << o.megas << std::endl
+        << "Number of Replicas " << o.R << std::endl
+        << "Number of Threads " << o.T << std::endl;
+    std::vector<double> betas(o.R);
+    if (o.tmin > 0 && o.tmax > 0) {
+        for (int l = 0; l < o.R; ++l)
+            betas[l] = 1.0 / (o.tmax * std::pow(o.tmin / o.tmax, o.R > 1 ? (double)l / (o.R - 1) : 1.0));
+    } else {
+        double b0 = o.beta0_ln, m = o.ratio;
+        if (!o.have_beta) {
+            log << "Automatic mode enabled, Beta parameters will be determined automatically." << std::endl;
+            if (!auto_select(o, n, F, D, b0, m, log)) return 1;
+        }
+        for (int l = 0; l < o.R; ++l) betas[l] = b0 / std::pow(m, o.R - 1 - l);  // CPU_PIM.cpp:113-118
+        log << "Maximum Beta " << b0 * 1.44269504089 << std::endl
+            << "Multiplicative Beta between iterations " << m << std::endl;
+        std::cout << "Maximum Beta " << b0 * 1.44269504089 << std::endl
+                  << "Multiplicative Beta between iterations " << m << std::endl;
+    }
+    log << "Input file: " << o.file << std::endl;
+
+    qap_problem prob{n, F.data(), D.data()};
+    qap_pt_params par{};
+    par.R = o.R;
+    par.T = o.T;
+    par.beta_ln = betas.data();
+    par.sweeps = (int64_t)o.megas * o.numitera;
+    par.sweeps_per_exchange = o.numitera;
+    par.seed = o.seed;
+    par.kick_after = 100;
+    par.has_goal = o.have_goal ? 1 : 0;
+    par.goal = o.goal;
+    par.timeout_s = o.timeout;
+    qap_pt_pool *pool = nullptr;
+    if (check(qap_pt_pool_create(1, &prob, &par, &pool))) return 1;
+    std::vector<int32_t> best_perm(n);
+    qap_pt_result res{};
+    res.best_perm = best_perm.data();
+    int64_t shown = INT64_MAX;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(o.megas, 100));
+    for (int64_t m = 0; m < o.megas; m += chunk) {
+        const int64_t step = std::min<int64_t>(chunk, o.megas - m);
+        if (check(qap_pt_pool_run(pool, step * o.numitera, nullptr, nullptr))) return 1;
+        if (check(qap_pt_pool_result(pool, 0, &res))) return 1;
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (res.best_cost < shown) {
+            shown = res.best_cost;
+            log << "Best Energy Found: " << shown << " Time (s): " << el << std::endl;
+            std::cout << "Best Energy Found: " << shown << " Time (s): " << el << std::endl;
+            std::ofstream sw(o.outdir + "/sweep0.txt", std::ios::app);
+            sw << m + step << " " << (long long)(el * 1000) << " " << shown << std::endl;
+        }
+        if (res.sweeps_done < (m + step) * o.numitera) break;  // goal or timeout
+        if (o.timeout > 0 && el >= o.timeout) break;
+    }
+    qap_pt_pool_destroy(pool);
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (o.goal_print > 0 && res.best_cost < o.goal_print) {
+        for (int i = 0; i < n; ++i) {
+            log << best_perm[i] << " ";
+            std::cout << best_perm[i] << " ";
+        }
+        log << std::endl;
+        std::cout << std::endl;
+    }
+    std::cout << std::endl << "Halting program." << std::endl;
+    log << "Single best energy found: " << res.best_cost << std::endl;
+    std::cout << "Single best energy found: " << res.best_cost << std::endl;
+    if (o.have_goal && o.goal != 0)
+        std::cout << "AFD " << (1.0 * res.best_cost - o.goal) / o.goal * 100.0 << std::endl;
+    log << "Elapsed time: " << el << " seconds." << std::endl;
+    std::cout << "Elapsed time: " << el << " seconds." << std::endl;
+    return 0;
+}

@@ -98,10 +98,9 @@ int round4i(int x) { return (x + 3) & ~3; }
 
 // Row strides of the force matrix.  Every row has at least one padding column (index
 // n): lanes without a column of their own in the rank-2 update read and rewrite it, so
-// full row groups need no per-lane predication.  The int32 LDS copy has 16-byte rows
-// (an odd number of 4-word quads) so the update runs on 128-bit LDS accesses; the
-// uint16 copy uses an odd word stride (ld_s / 2 words) so the column gather
-// I[pinv[b]][a] spreads over all banks.
+// full row groups need no per-lane predication.  Both LDS copies have 16-byte rows (an
+// odd number of 4-word quads for int32, of 8-column octets for uint16) so the rank-2
+// update runs on 128-bit LDS accesses.
 int hbm_stride(int n) { return round4i(n + 1); }
 
 int lds_stride(int n, int im) {
@@ -110,9 +109,9 @@ int lds_stride(int n, int im) {
         if (q % 2 == 0) ++q;
         return 4 * q;
     }
-    int ld = (n + 2) & ~1;
-    if ((ld / 2) % 2 == 0) ld += 2;
-    return ld;
+    int q8 = (n + 1 + 7) / 8;  // uint16: 16-byte rows, an odd number of 8-column octets
+    if (q8 % 2 == 0) ++q8;
+    return 8 * q8;
 }
 
 // Threads x candidates per thread and force-matrix placement for size n.
@@ -978,6 +977,17 @@ int qap_pt_solve_batch_hip(int32_t count, const qap_problem *problems,
 int qap_pt_solve_hip(const qap_problem *prob, const qap_pt_params *par, qap_pt_result *res) {
     if (!prob || !par || !res) return fail(QAP_PT_E_INVAL, "NULL argument");
     return qap_pt_solve_batch_hip(1, prob, par, res);
+}
+
+int qap_pt_pool_shape(qap_pt_pool *p, int32_t inst, char *buf, int32_t buflen) {
+    if (!p || !buf || buflen < 1) return fail(QAP_PT_E_INVAL, "bad argument");
+    if (inst < 0 || inst >= p->count) return fail(QAP_PT_E_INVAL, "instance %d out of range", inst);
+    const SweepShape &s = p->shapes[inst];
+    const InstDesc &d = p->desc[inst];
+    std::snprintf(buf, buflen, "%dx%d,%s,%s,%s", s.nt, s.cpl,
+                  s.im == IM_LDS32 ? "lds32" : (s.im == IM_LDS16 ? "lds16" : "hbm"),
+                  s.sym ? "sym" : "general", d.sparse_upd ? "sparse-flow" : "dense-flow");
+    return QAP_PT_OK;
 }
 
 int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen) {

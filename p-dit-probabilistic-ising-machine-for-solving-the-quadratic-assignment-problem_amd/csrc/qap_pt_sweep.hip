@@ -64,6 +64,7 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t u) {
 __device__ __forceinline__ int lane63(int v) { return __builtin_amdgcn_readlane(v, kWave - 1); }
 
 __host__ __device__ constexpr int round4(int x) { return (x + 3) & ~3; }
+__host__ __device__ constexpr int round8(int x) { return (x + 7) & ~7; }
 
 // Workgroup barrier that orders LDS only.  __syncthreads() also waits for every
 // outstanding global load (vmcnt(0)), which would stall the row prefetches; the sweep's
@@ -81,13 +82,15 @@ __device__ __forceinline__ void lds_sync() {
 }
 
 // red_s (ints): [0,16) max partials, [16,80) sum partials [q*16 + wave],
-// [96] winner b, [97] winner delta, [98] winner's partner location j.
-constexpr int kRedWords = 128;
-// p, pinv, cls, u1, u2, col, v1, v2, rnd, ddiag, drow x2, dcol x2, frow x2, ftrow x2
-constexpr int kSmallArrays = 18;
+// [96] winner b, [97] winner delta, [98] winner's partner location j,
+// [100,116) / [116,132) nonzero-column min / max partials.
+constexpr int kRedWords = 160;
+// p, pinv, cls, u1, u2, col, v1, v2, rnd, ddiag, drow x2, dcol x2, frow x2, ftrow x2,
+// vd1, vd2 -- each round8(n) words
+constexpr int kSmallArrays = 20;
 
 __host__ __device__ inline int sweep_lds_words(int n, int ld_s, int im) {
-    const int small = kSmallArrays * round4(n) + kRedWords;
+    const int small = kSmallArrays * round8(n) + kRedWords;
     if (im == IM_LDS32) return small + n * ld_s;
     if (im == IM_LDS16) return small + round4((n * ld_s + 1) / 2);
     return small;
@@ -155,21 +158,33 @@ __device__ __forceinline__ void dense_update(IT *const colp[CPL], int32_t rs, in
     }
 }
 
-// Dense rank-2 update of an int32 LDS force matrix with 16-byte rows (ld4 quads per
-// row): each lane owns one 4-column quad and a fixed residue of rows, reads and writes
-// it with 128-bit LDS accesses (a single wave issues one LDS instruction per ~23 cycles,
-// so 4 columns per access is what bounds this loop), four rows in flight.
-template <int NT, bool SYM, bool M24>
-__device__ __forceinline__ void quad_update(int4 *I4, int ld4, int n, const int32_t *u1_s,
-                                            const int32_t *u2_s, const int32_t *v1_s,
-                                            const int32_t *v2_s, int tid) {
-    const int Qn = (n + 3) >> 2;                  // quads holding real columns
-    const int rpp = NT / Qn;                      // rows per pass (>= 1: n <= 4 NT)
-    const int cx = tid % Qn, ry = tid / Qn;
+// Rank-2 update over the column range of vectors [V0, V1) of an LDS force matrix with
+// 16-byte rows: W = 4 int32 or W = 8 uint16 (holding -I) columns per 128-bit access.
+// Each lane owns one vector column and a fixed residue of rows; four rows in flight.
+// vd1/vd2 hold v for every column (zero outside the nonzero set).
+template <int NT, int IM, bool SYM, bool M24>
+__device__ __forceinline__ void vec_update(int4 *I4, int ldv, int n, int V0, int V1,
+                                           const int32_t *u1_s, const int32_t *u2_s,
+                                           const int32_t *vd1, const int32_t *vd2, int tid) {
+    constexpr int W = (IM == IM_LDS16) ? 8 : 4;
+    const int Vn = V1 - V0;
+    if (Vn <= 0) return;
+    const int rpp = NT / Vn;  // >= 1: Vn <= ceil(n / W) <= NT
+    const int cx = tid % Vn, ry = tid / Vn;
     if (ry >= rpp) return;
-    const int4 a1 = *reinterpret_cast<const int4 *>(v1_s + 4 * cx);
-    int4 a2 = int4{0, 0, 0, 0};
-    if constexpr (!SYM) a2 = *reinterpret_cast<const int4 *>(v2_s + 4 * cx);
+    const int vi = V0 + cx;
+    int32_t a1[W], a2[W];
+#pragma unroll
+    for (int t4 = 0; t4 < W; t4 += 4) {
+        const int4 x = *reinterpret_cast<const int4 *>(vd1 + vi * W + t4);
+        a1[t4] = x.x; a1[t4 + 1] = x.y; a1[t4 + 2] = x.z; a1[t4 + 3] = x.w;
+        if constexpr (!SYM) {
+            const int4 y = *reinterpret_cast<const int4 *>(vd2 + vi * W + t4);
+            a2[t4] = y.x; a2[t4 + 1] = y.y; a2[t4 + 2] = y.z; a2[t4 + 3] = y.w;
+        } else {
+            a2[t4] = a2[t4 + 1] = a2[t4 + 2] = a2[t4 + 3] = 0;
+        }
+    }
     constexpr int G = 4;
     for (int k0 = ry; k0 < n; k0 += G * rpp) {
         int4 x[G];
@@ -177,26 +192,35 @@ __device__ __forceinline__ void quad_update(int4 *I4, int ld4, int n, const int3
 #pragma unroll
         for (int r = 0; r < G; ++r) {
             const int k = min(k0 + r * rpp, n - 1);
-            x[r] = I4[k * ld4 + cx];
+            x[r] = I4[k * ldv + vi];
             w1[r] = u1_s[k];
-            if constexpr (!SYM) w2[r] = u2_s[k];
+            w2[r] = SYM ? 0 : u2_s[k];
         }
 #pragma unroll
         for (int r = 0; r < G; ++r) {
             const int k = k0 + r * rpp;
             if (k < n) {
-                int4 y = x[r];
-                y.x += umul<M24>(w1[r], a1.x);
-                y.y += umul<M24>(w1[r], a1.y);
-                y.z += umul<M24>(w1[r], a1.z);
-                y.w += umul<M24>(w1[r], a1.w);
-                if constexpr (!SYM) {
-                    y.x += umul<M24>(w2[r], a2.x);
-                    y.y += umul<M24>(w2[r], a2.y);
-                    y.z += umul<M24>(w2[r], a2.z);
-                    y.w += umul<M24>(w2[r], a2.w);
+                int32_t e[4] = {x[r].x, x[r].y, x[r].z, x[r].w};
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    if constexpr (IM == IM_LDS16) {
+                        // two uint16 (-I) per word: subtract the increments, wrap to 16 bits
+                        const int t = 2 * h;
+                        int32_t d0 = umul<M24>(w1[r], a1[t]), d1 = umul<M24>(w1[r], a1[t + 1]);
+                        if constexpr (!SYM) {
+                            d0 += umul<M24>(w2[r], a2[t]);
+                            d1 += umul<M24>(w2[r], a2[t + 1]);
+                        }
+                        const uint32_t lo = ((uint32_t)e[h] - (uint32_t)d0) & 0xffffu;
+                        const uint32_t hi = (((uint32_t)e[h] >> 16) - (uint32_t)d1) & 0xffffu;
+                        e[h] = (int32_t)(lo | (hi << 16));
+                    } else {
+                        int32_t dv = umul<M24>(w1[r], a1[h]);
+                        if constexpr (!SYM) dv += umul<M24>(w2[r], a2[h]);
+                        e[h] += dv;
+                    }
                 }
-                I4[k * ld4 + cx] = y;
+                I4[k * ldv + vi] = int4{e[0], e[1], e[2], e[3]};
             }
         }
     }
@@ -222,7 +246,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     if (!P.state[inst].active) return;
     const InstDesc d = P.inst[inst];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
-    const int n = d.n, ld = d.ld, ld_s = d.ld_s, n4 = round4(n);
+    const int n = d.n, ld = d.ld, ld_s = d.ld_s, n4 = round8(n);  // array stride
     const int local = g - d.rep_base;
 
     int32_t *p_s = smem;
@@ -239,7 +263,9 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     int32_t *ddiag_s = dcol_s + 2 * n4;
     int32_t *frow_s = ddiag_s + n4;                              // 2 buffers
     int32_t *ftrow_s = frow_s + 2 * n4;                          // 2 buffers
-    int32_t *red_s = ftrow_s + 2 * n4;
+    int32_t *vd1_s = ftrow_s + 2 * n4;                           // v of every column
+    int32_t *vd2_s = vd1_s + n4;
+    int32_t *red_s = vd2_s + n4;
     IT *I_s = reinterpret_cast<IT *>(red_s + kRedWords);
 
     const int32_t *__restrict__ F = P.F + d.mat_off;
@@ -295,29 +321,22 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                 }
             }
         }
-    } else if constexpr (IM == IM_LDS16) {
+    } else if constexpr (IM == IM_LDS16) {  // 4 int32 -> 4 uint16 (-I), 8-byte LDS stores
         const int4 *Ig4 = reinterpret_cast<const int4 *>(Ig);
+        uint2 *Is2 = reinterpret_cast<uint2 *>(I_s);
+        const int ld2 = ld_s >> 2;
         for (int cx = cx_st; cx < L4; cx += cw_st) {
             for (int k0 = ry_st; k0 < n; k0 += rw_st * kStageU) {
                 int4 v[kStageU];
 #pragma unroll
-                for (int u = 0; u < kStageU; ++u) {
-                    const int k = min(k0 + u * rw_st, n - 1);
-                    v[u] = Ig4[(int64_t)k * L4 + cx];
-                }
+                for (int u = 0; u < kStageU; ++u) v[u] = Ig4[(int64_t)min(k0 + u * rw_st, n - 1) * L4 + cx];
 #pragma unroll
                 for (int u = 0; u < kStageU; ++u) {
                     const int k = k0 + u * rw_st;
                     if (k < n) {
-                        const int c = 4 * cx;
-                        const int32_t e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-#pragma unroll
-                        for (int t = 0; t < 4; ++t) {
-                            if (c + t < n) {
-                                if constexpr (IM == IM_LDS16) I_s[k * ld_s + c + t] = (uint16_t)(-e[t]);
-                                else I_s[k * ld_s + c + t] = e[t];
-                            }
-                        }
+                        const uint32_t w0 = ((uint32_t)(-v[u].x) & 0xffffu) | ((uint32_t)(-v[u].y) << 16);
+                        const uint32_t w1 = ((uint32_t)(-v[u].z) & 0xffffu) | ((uint32_t)(-v[u].w) << 16);
+                        Is2[k * ld2 + cx] = uint2{w0, w1};
                     }
                 }
             }
@@ -587,31 +606,51 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                         }
                     }
                     uint32_t nnz = (uint32_t)n;
-                    if (IM == IM_LDS32 && !sparse_upd) {  // quad update reads v from LDS
+                    int c_lo = 0, c_hi = n - 1;
+                    if constexpr (IM != IM_HBM) {  // v of every column for the vector path
 #pragma unroll
                         for (int q = 0; q < CPL; ++q) {
                             const int c = tid + q * NT;
                             if (c < n4) {
-                                v1_s[c] = v1[q];
-                                if constexpr (!SYM) v2_s[c] = v2[q];
+                                vd1_s[c] = v1[q];
+                                if constexpr (!SYM) vd2_s[c] = v2[q];
                             }
                         }
                     }
                     if (sparse_upd) {
+                        // count, range and compaction of the nonzero columns
+                        int lo = n, hi = -1;
+#pragma unroll
+                        for (int q = 0; q < CPL; ++q)
+                            if ((v1[q] | v2[q]) != 0) {
+                                lo = min(lo, tid + q * NT);
+                                hi = max(hi, tid + q * NT);
+                            }
                         const uint32_t inc = wave_incl_sum(nz);
+                        const int wlo = -wave_incl_max(-lo), whi = wave_incl_max(hi);
                         uint32_t base;
                         if constexpr (NW == 1) {
                             base = inc - nz;
                             nnz = (uint32_t)lane63((int)inc);
+                            c_lo = lane63(wlo);
+                            c_hi = lane63(whi);
                         } else {
-                            if (lane == kWave - 1) red_s[16 + wid] = (int)inc;
+                            if (lane == kWave - 1) {
+                                red_s[16 + wid] = (int)inc;
+                                red_s[100 + wid] = wlo;
+                                red_s[116 + wid] = whi;
+                            }
                             lds_sync<NW>();
                             uint32_t before = 0, tot = 0;
+                            c_lo = n;
+                            c_hi = -1;
 #pragma unroll
                             for (int ww = 0; ww < NW; ++ww) {
                                 const uint32_t x = (uint32_t)red_s[16 + ww];
                                 before += (ww < wid) ? x : 0u;
                                 tot += x;
+                                c_lo = min(c_lo, red_s[100 + ww]);
+                                c_hi = max(c_hi, red_s[116 + ww]);
                             }
                             base = before + inc - nz;
                             nnz = tot;
@@ -627,13 +666,21 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                         }
                     }
                     lds_sync<NW>();
-                    cells += (uint64_t)n * nnz;
-                    if (IM == IM_LDS32 && !sparse_upd) {
+                    constexpr int VW = (IM == IM_LDS16) ? 8 : 4;
+                    const bool vec_path =
+                        IM != IM_HBM && (!sparse_upd || c_hi - c_lo + 1 <= 4 * (int)nnz + 4 * VW);
+                    cells += (uint64_t)n * (vec_path ? (uint32_t)(c_hi - c_lo + 1) : nnz);
+                    if (nnz == 0) {
+                        // flow differences all zero: nothing to update
+                    } else if (vec_path) {
                         int4 *I4 = reinterpret_cast<int4 *>(I_s);
+                        const int V0 = c_lo / VW, V1 = c_hi / VW + 1;
                         if (d.mul24)
-                            quad_update<NT, SYM, true>(I4, ld_s >> 2, n, u1_s, u2_s, v1_s, v2_s, tid);
+                            vec_update<NT, IM, SYM, true>(I4, ld_s / VW, n, V0, V1, u1_s, u2_s,
+                                                          vd1_s, vd2_s, tid);
                         else
-                            quad_update<NT, SYM, false>(I4, ld_s >> 2, n, u1_s, u2_s, v1_s, v2_s, tid);
+                            vec_update<NT, IM, SYM, false>(I4, ld_s / VW, n, V0, V1, u1_s, u2_s,
+                                                           vd1_s, vd2_s, tid);
                     } else if (!sparse_upd) {
                         IT *colp[CPL];
 #pragma unroll
@@ -712,20 +759,19 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
             for (int k = ry_st; k < n; k += rw_st) Ig4[(int64_t)k * L4 + cx] = Is4[k * ld4 + cx];
     } else if constexpr (IM == IM_LDS16) {
         int4 *Ig4 = reinterpret_cast<int4 *>(Ig);
-        for (int cx = cx_st; cx < L4; cx += cw_st) {
+        const uint2 *Is2 = reinterpret_cast<const uint2 *>(I_s);
+        const int ld2 = ld_s >> 2;
+        for (int cx = cx_st; cx < L4; cx += cw_st)
             for (int k = ry_st; k < n; k += rw_st) {
+                const uint2 w = Is2[k * ld2 + cx];
                 const int c = 4 * cx;
-                int32_t e[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int cc = min(c + t, n - 1);
-                    const int32_t v = (IM == IM_LDS16) ? -(int32_t)I_s[k * ld_s + cc]
-                                                       : (int32_t)I_s[k * ld_s + cc];
-                    e[t] = (c + t < n) ? v : 0;
-                }
-                Ig4[(int64_t)k * L4 + cx] = int4{e[0], e[1], e[2], e[3]};
+                int4 o;
+                o.x = (c + 0 < n) ? -(int32_t)(w.x & 0xffffu) : 0;
+                o.y = (c + 1 < n) ? -(int32_t)(w.x >> 16) : 0;
+                o.z = (c + 2 < n) ? -(int32_t)(w.y & 0xffffu) : 0;
+                o.w = (c + 3 < n) ? -(int32_t)(w.y >> 16) : 0;
+                Ig4[(int64_t)k * L4 + cx] = o;
             }
-        }
     }
     if (tid == 0) {
         P.E[g] = E;
