@@ -163,6 +163,26 @@ int qap_srv_stats(qap_srv *srv, int64_t *queued, int64_t *running, int64_t *done
 /* Finishes running pools, fails queued requests with QAP_PT_E_BUSY, frees everything. */
 int qap_srv_destroy(qap_srv *srv);
 
+/* --- automatic ladder (CPU_PIM.cpp:366-476: autoSelectParameters) -----------------
+ * Exponents b = QAP_PT_AUTO_MIN_EXP .. QAP_PT_AUTO_MIN_EXP + QAP_PT_AUTO_STEPS - 1; at
+ * each, QAP_PT_AUTO_TRIALS runs of base->T ladders x base->R replicas, all at exp2-unit
+ * beta 2^b, for QAP_PT_AUTO_MEGAS exchange intervals of base->sweeps_per_exchange
+ * sweeps.  avg_energy[i] (may be NULL, else QAP_PT_AUTO_STEPS doubles) receives the mean
+ * final cost at exponent i; beta0_ln and ratio the ladder for qap_pt_params as the
+ * reference's -b / -m would give it (largest beta, multiplicative step).  base->beta_ln,
+ * sweeps, has_goal, goal, timeout_s and init_perm are ignored. */
+#define QAP_PT_AUTO_MIN_EXP (-30)
+#define QAP_PT_AUTO_STEPS 32
+#define QAP_PT_AUTO_TRIALS 5
+#define QAP_PT_AUTO_MEGAS 10
+int qap_pt_auto_beta(const qap_problem *prob, const qap_pt_params *base, double lower_prop,
+                     double higher_prop, double *avg_energy, double *beta0_ln, double *ratio);
+/* The selection rule alone (CPU_PIM.cpp:431-473) over avg_energy[steps] at exponents
+ * QAP_PT_AUTO_MIN_EXP + i; lower_prop / higher_prop are the reference's -al / -ah
+ * (defaults 0.1 / 0.9).  Host only, no device needed. */
+int qap_pt_auto_select(const double *avg_energy, int32_t steps, double lower_prop,
+                       double higher_prop, int32_t R, double *beta0_ln, double *ratio);
+
 /* Sweep-kernel shape chosen for a symmetric instance of size n with a large force
  * bound ("threads x candidates per thread, lds32|lds16|hbm"), for reports. */
 int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen);

@@ -28,6 +28,8 @@ QAP_PT_E_OVERFLOW = -4
 QAP_PT_E_BUSY = -5
 QAP_PT_INIT_PER_REPLICA = 1
 MAX_N = 1024
+AUTO_MIN_EXP = -30
+AUTO_STEPS = 32
 
 # Every symbol the C header declares (checked by the CPU test suite).
 EXPORTS = (
@@ -43,6 +45,8 @@ EXPORTS = (
     "qap_pt_pool_destroy",
     "qap_pt_kernel_shape",
     "qap_pt_pool_shape",
+    "qap_pt_auto_beta",
+    "qap_pt_auto_select",
     "qap_srv_create",
     "qap_srv_submit",
     "qap_srv_poll",
@@ -141,6 +145,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.qap_pt_pool_destroy.argtypes = [C.c_void_p]
     lib.qap_pt_kernel_shape.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
     lib.qap_pt_pool_shape.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]
+    lib.qap_pt_auto_beta.argtypes = [C.POINTER(QapProblem), C.POINTER(QapPtParams), C.c_double,
+                                     C.c_double, _F64P, _F64P, _F64P]
+    lib.qap_pt_auto_select.argtypes = [_F64P, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                       _F64P, _F64P]
     lib.qap_srv_create.argtypes = [C.POINTER(QapSrvConfig), C.POINTER(C.c_void_p)]
     lib.qap_srv_submit.argtypes = [C.c_void_p, C.POINTER(QapProblem), C.POINTER(QapPtParams),
                                    C.POINTER(C.c_uint64)]
@@ -220,6 +228,44 @@ class Instance:
         par.init_perm = (_ptr(self.init_perm, C.c_int32) if self.init_perm is not None
                          else C.cast(None, _I32P))
         return prob, par
+
+
+def auto_select(avg_energy, R: int, lower_prop: float = 0.1,
+                higher_prop: float = 0.9) -> tuple[float, float]:
+    """qap_pt_auto_select: the reference's ladder rule (CPU_PIM.cpp:431-473) over mean
+    final costs at exp2-unit betas 2^(AUTO_MIN_EXP + i).  Returns (beta0_ln, ratio)."""
+    avg = np.ascontiguousarray(avg_energy, dtype=np.float64)
+    b0, m = C.c_double(), C.c_double()
+    _check(load_library().qap_pt_auto_select(_ptr(avg, C.c_double), avg.size, lower_prop,
+                                             higher_prop, R, C.byref(b0), C.byref(m)))
+    return b0.value, m.value
+
+
+def auto_beta(F, D, R: int, T: int = 1, sweeps_per_exchange: int = 1, seed: int = 0,
+              lower_prop: float = 0.1, higher_prop: float = 0.9, per_replica_init=False):
+    """qap_pt_auto_beta: the automatic ladder scan (CPU_PIM.cpp:366-476) as one batched
+    run.  Returns (beta0_ln, ratio, avg_energy[AUTO_STEPS]); the ladder is then
+    beta_ln[l] = beta0_ln / ratio**(R - 1 - l)."""
+    inst = Instance(F=F, D=D, beta_ln=np.ones(R), sweeps=0, T=T,
+                    sweeps_per_exchange=sweeps_per_exchange, seed=seed,
+                    per_replica_init=per_replica_init)
+    prob, par = inst.c_structs()
+    avg = np.zeros(AUTO_STEPS, dtype=np.float64)
+    b0, m = C.c_double(), C.c_double()
+    _check(load_library().qap_pt_auto_beta(C.byref(prob), C.byref(par), lower_prop, higher_prop,
+                                           _ptr(avg, C.c_double), C.byref(b0), C.byref(m)))
+    return b0.value, m.value, avg
+
+
+def ladder_from(beta0_ln: float, ratio: float, R: int) -> np.ndarray:
+    """The reference's ladder (CPU_PIM.cpp:113-118): replica R-1 at beta0, each colder
+    rung divided by ratio in turn; returned hottest first."""
+    out = np.zeros(R, dtype=np.float64)
+    b = float(beta0_ln)
+    for l in range(R - 1, -1, -1):
+        out[l] = b
+        b /= ratio
+    return out
 
 
 def _result_dict(res: QapPtResult, perm: np.ndarray) -> dict:

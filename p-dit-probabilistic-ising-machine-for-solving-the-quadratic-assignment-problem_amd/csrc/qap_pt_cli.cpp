@@ -76,77 +76,35 @@ int check(int rc) {
     return rc;
 }
 
-// Automatic ladder (CPU_PIM.cpp:366-476): for each exponent b in [-30, 1] run 5 trials of
-// T ladders x R replicas, all at beta = 2^b (exp2 units), for 10 exchange intervals from
-// fresh shuffles, and average the final cost of every replica.  The whole sweep of
-// 32 x 5 trials is one batched pool.  The selection rule is the reference's.
+// Automatic ladder (CPU_PIM.cpp:366-476) through qap_pt_auto_beta, with the
+// reference's log lines.
 bool auto_select(const Options &o, int n, const std::vector<int32_t> &F, const std::vector<int32_t> &D,
                  double &beta0_ln, double &ratio, std::ostream &log) {
-    const double kMin = -30, kMax = 1, kStep = 1;
-    const int steps = (int)((kMax - kMin) / kStep) + 1, trials = 5, megas = 10;
-    std::vector<qap_problem> probs;
-    std::vector<qap_pt_params> pars;
-    std::vector<std::vector<double>> ladders;
-    for (int i = 0; i < steps; ++i) {
-        const double b2 = std::exp2(kMin + i * kStep);
-        ladders.emplace_back(o.R, b2 / 1.44269504089);  // beta_ln with the reference constant
+    qap_problem prob{n, F.data(), D.data()};
+    qap_pt_params base{};
+    base.R = o.R;
+    base.T = o.T;
+    base.sweeps_per_exchange = o.numitera;
+    base.seed = o.seed;
+    base.kick_after = 100;
+    const auto t0 = std::chrono::steady_clock::now();
+    double avg[QAP_PT_AUTO_STEPS];
+    if (check(qap_pt_auto_beta(&prob, &base, o.lower_prop, o.higher_prop, avg, &beta0_ln, &ratio)))
+        return false;
+    for (int i = 0; i < QAP_PT_AUTO_STEPS; ++i) {
+        const double b = std::exp2(QAP_PT_AUTO_MIN_EXP + i);
+        log << "Beta: " << b << ", Average Energy: " << avg[i] << std::endl;
+        std::cout << "Beta: " << b << ", Average Energy: " << avg[i] << std::endl;
     }
-    for (int i = 0; i < steps; ++i)
-        for (int t = 0; t < trials; ++t) {
-            probs.push_back(qap_problem{n, F.data(), D.data()});
-            qap_pt_params p{};
-            p.R = o.R;
-            p.T = o.T;
-            p.beta_ln = ladders[i].data();
-            p.sweeps = (int64_t)megas * o.numitera;
-            p.sweeps_per_exchange = o.numitera;
-            p.seed = o.seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(1 + i * trials + t));
-            p.kick_after = 100;
-            pars.push_back(p);
-        }
-    qap_pt_pool *pool = nullptr;
-    if (check(qap_pt_pool_create((int32_t)probs.size(), probs.data(), pars.data(), &pool))) return false;
-    if (check(qap_pt_pool_run(pool, -1, nullptr, nullptr))) return false;
-    std::vector<double> avg(steps, 0.0);
-    std::vector<int64_t> E((size_t)o.R * o.T);
-    for (int i = 0; i < steps; ++i)
-        for (int t = 0; t < trials; ++t) {
-            if (check(qap_pt_pool_state(pool, i * trials + t, nullptr, E.data(), nullptr, nullptr,
-                                        nullptr, nullptr)))
-                return false;
-            for (auto e : E) avg[i] += (double)e / ((double)trials * o.T * o.R);
-        }
-    qap_pt_pool_destroy(pool);
-    for (int i = 0; i < steps; ++i) {
-        log << "Beta: " << std::exp2(kMin + i * kStep) << ", Average Energy: " << avg[i] << std::endl;
-        std::cout << "Beta: " << std::exp2(kMin + i * kStep) << ", Average Energy: " << avg[i] << std::endl;
-    }
-    // CPU_PIM.cpp:434-473 (evaluated once the last exponent has run)
-    const int last = steps - 1;
-    double minE = 1e30, maxE = -1e30, minBeta = 0, maxBeta = 0;
-    int minIdx = 0;
-    for (int j = 0; j < last; ++j) {
-        if (avg[j] < minE) {
-            minE = avg[j];
-            minIdx = j;
-        }
-        if (j == 0) maxE = avg[j];
-    }
-    for (int j = 0; j < last; ++j) {
-        if (minBeta == 0 && ((j == minIdx - 1 && j >= 1) || avg[j] < (maxE - minE) * o.higher_prop + minE))
-            minBeta = std::exp2(kMin + j * kStep);
-        if (j == last - 1 ||
-            (minBeta != 0 && (j == minIdx || avg[j] < (maxE - minE) * o.lower_prop + minE))) {
-            if (minBeta == 0) minBeta = std::exp2(kMin);
-            maxBeta = std::exp2(kMin + (j + 2) * kStep);
-            break;
-        }
-    }
-    log << "Selecting minimum beta of " << minBeta << std::endl << "Selecting maximum beta of " << maxBeta << std::endl;
+    const double maxBeta = beta0_ln * 1.44269504089, minBeta = maxBeta / std::pow(ratio, o.R);
+    const long long secs = (long long)std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    log << "Selecting minimum beta of " << minBeta << std::endl
+        << "Selecting maximum beta of " << maxBeta << std::endl
+        << "Selecting Beta Parameters took " << secs << " seconds." << std::endl;
     std::cout << "Selecting minimum beta of " << minBeta << std::endl
-              << "Selecting maximum beta of " << maxBeta << std::endl;
-    beta0_ln = maxBeta / 1.44269504089;
-    ratio = std::pow(maxBeta / minBeta, 1.0 / o.R);
+              << "Selecting maximum beta of " << maxBeta << std::endl
+              << "Selecting Beta Parameters took " << secs << " seconds." << std::endl
+              << std::endl;
     return true;
 }
 
@@ -220,7 +178,8 @@ int main(int argc, char **argv) {
             log << "Automatic mode enabled, Beta parameters will be determined automatically." << std::endl;
             if (!auto_select(o, n, F, D, b0, m, log)) return 1;
         }
-        for (int l = 0; l < o.R; ++l) betas[l] = b0 / std::pow(m, o.R - 1 - l);  // CPU_PIM.cpp:113-118
+        double run = b0;  // CPU_PIM.cpp:113-118: rung R-1 at beta0, each colder one divided by m
+        for (int l = o.R - 1; l >= 0; --l, run /= m) betas[l] = run;
         log << "Maximum Beta " << b0 * 1.44269504089 << std::endl
             << "Multiplicative Beta between iterations " << m << std::endl;
         std::cout << "Maximum Beta " << b0 * 1.44269504089 << std::endl
