@@ -115,6 +115,9 @@ int qap_pt_pool_create(int32_t count, const qap_problem *problems, const qap_pt_
  * the number of such launches are returned. */
 int qap_pt_pool_run(qap_pt_pool *pool, int64_t sweeps, double *kernel_ms, int64_t *launches);
 int qap_pt_pool_result(qap_pt_pool *pool, int32_t inst, qap_pt_result *res);
+/* best_costs[count]: every instance's best cost so far, as qap_pt_pool_result reports
+ * it, with one device copy for the whole pool (CUPIM's thunk_scores, qap.cu:28-303). */
+int qap_pt_pool_best_costs(qap_pt_pool *pool, int64_t *best_costs);
 /* Copies instance inst's replica state (each pointer may be NULL). Sizes use
  * RT = R*T: perm RT*n, E RT, bestE RT (INT64_MAX = no accepted move yet), pos RT
  * (ladder position, 0 = hottest), accepted RT, best_perms RT*n. */

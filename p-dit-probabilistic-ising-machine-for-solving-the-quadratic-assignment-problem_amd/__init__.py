@@ -40,6 +40,7 @@ EXPORTS = (
     "qap_pt_pool_create",
     "qap_pt_pool_run",
     "qap_pt_pool_result",
+    "qap_pt_pool_best_costs",
     "qap_pt_pool_state",
     "qap_pt_pool_verify",
     "qap_pt_pool_destroy",
@@ -141,6 +142,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.qap_pt_pool_result.argtypes = [C.c_void_p, C.c_int32, C.POINTER(QapPtResult)]
     lib.qap_pt_pool_state.argtypes = [C.c_void_p, C.c_int32, _I32P, _I64P, _I64P, _I32P,
                                       _U64P, _I32P]
+    lib.qap_pt_pool_best_costs.argtypes = [C.c_void_p, _I64P]
     lib.qap_pt_pool_verify.argtypes = [C.c_void_p, _I64P, _I64P]
     lib.qap_pt_pool_destroy.argtypes = [C.c_void_p]
     lib.qap_pt_kernel_shape.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
@@ -260,12 +262,7 @@ def auto_beta(F, D, R: int, T: int = 1, sweeps_per_exchange: int = 1, seed: int 
 def ladder_from(beta0_ln: float, ratio: float, R: int) -> np.ndarray:
     """The reference's ladder (CPU_PIM.cpp:113-118): replica R-1 at beta0, each colder
     rung divided by ratio in turn; returned hottest first."""
-    out = np.zeros(R, dtype=np.float64)
-    b = float(beta0_ln)
-    for l in range(R - 1, -1, -1):
-        out[l] = b
-        b /= ratio
-    return out
+    return instances.beta_ladder_from_flags(R, beta0_ln, ratio)
 
 
 def _result_dict(res: QapPtResult, perm: np.ndarray) -> dict:
@@ -340,6 +337,12 @@ class Pool:
         res.best_perm = _ptr(perm, C.c_int32)
         _check(self.lib.qap_pt_pool_result(self.handle, k, C.byref(res)))
         return _result_dict(res, perm)
+
+    def best_costs(self) -> np.ndarray:
+        """Every instance's best cost so far (one device copy)."""
+        out = np.zeros(len(self.insts), dtype=np.int64)
+        _check(self.lib.qap_pt_pool_best_costs(self.handle, _ptr(out, C.c_int64)))
+        return out
 
     def state(self, k: int = 0) -> dict:
         inst = self.insts[k]

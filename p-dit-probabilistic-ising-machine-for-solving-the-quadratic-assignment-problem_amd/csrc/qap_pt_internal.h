@@ -22,4 +22,20 @@ void pool_destroy_internal(qap_pt_pool *p);
 // Device bytes a pool of these instances allocates.
 int64_t pool_device_bytes(int32_t count, const qap_problem *problems, const qap_pt_params *params);
 
+// Instances whose off-diagonal flow density is below this use the sparse-flow update and
+// the "sparse" rows of the tuned shape table.
+constexpr double kSparseFlow = 0.35;
+
+// One row of the tuned shape table: instances with n <= n_max (and above the previous
+// row's n_max) of this flow class whose force matrix sits in placement im (IMode) use
+// nt threads x cpl candidates per thread; rate is the tuned swap-attempts/s (0 = none).
+struct TunedShape {
+    int n_max, sparse, im, nt, cpl;
+    double rate;
+};
+// The tuned row for an instance of size n in placement im, or nullptr.  The table is read once per
+// process from $QAP_PT_TUNE_TABLE (empty: no table) or else from
+// <directory of libqap_pt.so>/../tune/shapes_gfx950.txt; a missing file means no table.
+const TunedShape *tuned_shape(int n, bool sparse, int im);
+
 }  // namespace qap

@@ -115,10 +115,47 @@ int lds_stride(int n, int im) {
 }
 
 // Threads x candidates per thread and force-matrix placement for size n.
-// QAP_PT_SHAPE="nt,cpl" and QAP_PT_IMODE=0|1|2 override (tuning and tests).
-SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, int *ld_s) {
+// Placement: QAP_PT_IMODE=0|1|2 when that one fits (LDS budget, uint16 range); else the
+// fitting placement with the highest tuned rate when the table rates all of them; else
+// the first of lds32 -> lds16 -> hbm that fits.  Shape: QAP_PT_SHAPE="nt,cpl" (tests and
+// the autotuner), else the tuned table's row for (n, flow class, placement), else a size
+// rule.
+SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool sparse, int *ld_s) {
     SweepShape s{};
     s.sym = sym;
+    const bool fit32 = sweep_lds_bytes(n, lds_stride(n, IM_LDS32), IM_LDS32) <= kMaxLdsBytes;
+    const bool fit16 = nonneg && force_bound <= 65535 &&
+                       sweep_lds_bytes(n, lds_stride(n, IM_LDS16), IM_LDS16) <= kMaxLdsBytes;
+    const char *im_env = std::getenv("QAP_PT_IMODE");
+    const int want = im_env ? std::atoi(im_env) : -1;
+    if (want == IM_HBM) s.im = IM_HBM;
+    else if (want == IM_LDS16 && fit16) s.im = IM_LDS16;
+    else if (fit32) s.im = IM_LDS32;
+    else if (fit16) s.im = IM_LDS16;
+    else s.im = IM_HBM;
+    if (want < 0) {
+        // With tuned rates for every placement that fits, take the fastest (a uint16
+        // matrix halves the LDS footprint, so more replicas share a CU).
+        const int cand[3] = {fit32 ? IM_LDS32 : -1, fit16 ? IM_LDS16 : -1, IM_HBM};
+        double best_rate = 0.0;
+        int best_im = -1;
+        bool all_rated = true;
+        for (int im : cand) {
+            if (im < 0) continue;
+            const TunedShape *t = tuned_shape(n, sparse, im);
+            if (!t || !(t->rate > 0.0)) {
+                all_rated = false;
+                break;
+            }
+            if (t->rate > best_rate) {
+                best_rate = t->rate;
+                best_im = im;
+            }
+        }
+        if (all_rated && best_im >= 0) s.im = best_im;
+    }
+    *ld_s = lds_stride(n, s.im);
+
     if (n <= 64) {
         s.nt = 64;
         s.cpl = 1;
@@ -135,27 +172,17 @@ SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, int *
         s.nt = 256;
         s.cpl = 4;
     }
-    if (const char *env = std::getenv("QAP_PT_SHAPE")) {
-        int nt = 0, cpl = 0;
-        if (std::sscanf(env, "%d,%d", &nt, &cpl) == 2 && sweep_shape_supported(nt, cpl) &&
-            nt * cpl >= n) {
+    auto take = [&](int nt, int cpl) {
+        if (sweep_shape_supported(nt, cpl) && nt * cpl >= n) {
             s.nt = nt;
             s.cpl = cpl;
         }
+    };
+    if (const TunedShape *t = tuned_shape(n, sparse, s.im)) take(t->nt, t->cpl);
+    if (const char *env = std::getenv("QAP_PT_SHAPE")) {
+        int nt = 0, cpl = 0;
+        if (std::sscanf(env, "%d,%d", &nt, &cpl) == 2) take(nt, cpl);
     }
-    const bool fits16 = nonneg && force_bound <= 65535;
-    if (sweep_lds_bytes(n, lds_stride(n, IM_LDS32), IM_LDS32) <= kMaxLdsBytes) s.im = IM_LDS32;
-    else if (fits16 && sweep_lds_bytes(n, lds_stride(n, IM_LDS16), IM_LDS16) <= kMaxLdsBytes)
-        s.im = IM_LDS16;
-    else s.im = IM_HBM;
-    if (const char *env = std::getenv("QAP_PT_IMODE")) {
-        const int im = std::atoi(env);
-        if (im == IM_HBM) s.im = IM_HBM;
-        if (im == IM_LDS16 && fits16 &&
-            sweep_lds_bytes(n, lds_stride(n, IM_LDS16), IM_LDS16) <= kMaxLdsBytes)
-            s.im = IM_LDS16;
-    }
-    *ld_s = lds_stride(n, s.im);
     return s;
 }
 
@@ -400,12 +427,13 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
         for (int64_t x = 0; x < (int64_t)n * n && nonneg; ++x) nonneg = pb.F[x] >= 0 && pb.D[x] >= 0;
         const Bounds bnd = compute_bounds(n, pb.F, pb.D);
         int ld_s = 0;
-        const SweepShape sh = choose_shape(n, sym, nonneg, bnd.force, &ld_s);
+        const bool sparse = offdiag_density(n, pb.F) < kSparseFlow;
+        const SweepShape sh = choose_shape(n, sym, nonneg, bnd.force, sparse, &ld_s);
         p->shapes[k] = sh;
         d.n = n;
         d.ld = hbm_stride(n);
         d.ld_s = ld_s;
-        d.sparse_upd = offdiag_density(n, pb.F) < 0.35 ? 1 : 0;
+        d.sparse_upd = sparse ? 1 : 0;
         // u = distance differences (<= 2 max|D|), v = flow differences (<= 4 max|F|)
         d.mul24 = (2 * bnd.maxD < (1 << 23) && 4 * bnd.maxF < (1 << 23)) ? 1 : 0;
         d.R = pr.R;
@@ -894,6 +922,30 @@ int qap_pt_pool_result(qap_pt_pool *pool, int32_t inst, qap_pt_result *res) {
     }
 }
 
+int qap_pt_pool_best_costs(qap_pt_pool *p, int64_t *best_costs) {
+    if (!p || !best_costs) return fail(QAP_PT_E_INVAL, "NULL argument");
+    try {
+        HIP_TRY(hipSetDevice(p->device));
+        std::vector<int64_t> E(p->total_reps), bestE(p->total_reps);
+        HIP_TRY(hipMemcpy(E.data(), p->d_E, 8 * (size_t)p->total_reps, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(bestE.data(), p->d_bestE, 8 * (size_t)p->total_reps, hipMemcpyDeviceToHost));
+        for (int k = 0; k < p->count; ++k) {
+            const InstDesc &d = p->desc[k];
+            const int RT = d.R * d.T;
+            int64_t best = INT64_MAX, init = INT64_MAX;
+            for (int r = 0; r < RT; ++r) {
+                best = std::min(best, bestE[d.rep_base + r]);
+                init = std::min(init, E[d.rep_base + r]);
+            }
+            // as qap_pt_pool_result: with no accepted move anywhere, the best initial cost
+            best_costs[k] = best != INT64_MAX ? best : init;
+        }
+        return QAP_PT_OK;
+    } catch (const std::bad_alloc &) {
+        return fail(QAP_PT_E_NOMEM, "host allocation failed");
+    }
+}
+
 int qap_pt_pool_state(qap_pt_pool *p, int32_t inst, int32_t *perm, int64_t *E, int64_t *bestE,
                       int32_t *pos, uint64_t *accepted, int32_t *best_perms) {
     if (!p) return fail(QAP_PT_E_INVAL, "pool is NULL");
@@ -993,7 +1045,7 @@ int qap_pt_pool_shape(qap_pt_pool *p, int32_t inst, char *buf, int32_t buflen) {
 int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen) {
     if (n < 1 || n > QAP_PT_MAX_N || !buf || buflen < 1) return fail(QAP_PT_E_INVAL, "bad argument");
     int ld_s = 0;
-    const SweepShape s = choose_shape(n, true, true, (int64_t)1 << 40, &ld_s);
+    const SweepShape s = choose_shape(n, true, true, (int64_t)1 << 40, false, &ld_s);
     std::snprintf(buf, buflen, "%dx%d,%s", s.nt, s.cpl,
                   s.im == IM_LDS32 ? "lds32" : (s.im == IM_LDS16 ? "lds16" : "hbm"));
     return QAP_PT_OK;

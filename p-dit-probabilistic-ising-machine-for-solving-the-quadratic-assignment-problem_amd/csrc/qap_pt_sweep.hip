@@ -375,6 +375,9 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         pv[q] = (b < n) ? pinv_s[b] : 0;
     }
     int buf = 0;
+    // facility at the current site, carried in a register from update to update (the
+    // swap at site i moves it only when the partner location is the next site)
+    int a_reg = a_cur;
 
     for (int s = 0; s < d.spe; ++s) {
         const uint32_t sweep = sweep0 + (uint32_t)s;
@@ -386,7 +389,8 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         lds_sync<NW>();
 
         for (int i = 0; i < n; ++i) {
-            const int a = p_s[i];
+            const int a = a_reg;
+            const uint32_t ri = rnd_s[i];
             if (a != a_cur) {  // the previous swap moved the facility we prefetched
                 a_cur = a;
                 for (int x = tid; x < n; x += NT) {
@@ -494,7 +498,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
 
             // (3) winner: first b whose inclusive prefix exceeds floor(u * S / 2^32);
             //     it commits the permutation swap before the broadcast barrier.
-            const uint32_t thr = (uint32_t)(((uint64_t)rnd_s[i] * (uint64_t)S) >> 32);
+            const uint32_t thr = (uint32_t)(((uint64_t)ri * (uint64_t)S) >> 32);
             int my_b = a, my_j = i;
             int32_t my_d = 0;
             bool owner = false;
@@ -741,6 +745,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
             // the prefetched rows become current
             buf ^= 1;
             a_cur = a_nx;
+            a_reg = (b != a && j == inext) ? a : a_nx;
         }
     }
 

@@ -28,8 +28,13 @@ def geometric_ladder(R: int, t_min: float, t_max: float) -> np.ndarray:
 
 def beta_ladder_from_flags(R: int, beta0_ln: float, ratio: float) -> np.ndarray:
     """The reference's -b/-m ladder: largest beta beta0 (ln units), each rung below
-    divided by `ratio` (CPU_PIM.cpp:111-119)."""
-    return np.array([beta0_ln / ratio ** (R - 1 - l) for l in range(R)], dtype=np.float64)
+    divided by `ratio` in turn (CPU_PIM.cpp:111-119)."""
+    out = np.zeros(R, dtype=np.float64)
+    b = float(beta0_ln)
+    for l in range(R - 1, -1, -1):
+        out[l] = b
+        b /= ratio
+    return out
 
 
 def _sym_uniform(g: np.random.Generator, n: int, lo: int, hi: int) -> np.ndarray:

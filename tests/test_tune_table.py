@@ -1,0 +1,98 @@
+"""The tuned shape table: parsing and precedence (host only, no device).
+
+The library reads the table once per process, so each case runs in a fresh Python
+process with QAP_PT_TUNE_TABLE pointing at a file written here.
+"""
+import os
+import subprocess
+import sys
+import textwrap
+
+from conftest import ROOT
+
+PROBE = textwrap.dedent("""
+    import sys
+    sys.path.insert(0, {tests!r})
+    from conftest import load_qpt
+    q = load_qpt()
+    for n in (16, 40, 64, 65, 100, 129, 200, 300, 700):
+        print(n, q.kernel_shape(n))
+""")
+
+
+def _shapes(tmp_path, table_text, extra_env=None):
+    env = dict(os.environ)
+    env.pop("QAP_PT_SHAPE", None)
+    env.pop("QAP_PT_IMODE", None)
+    if table_text is None:
+        env["QAP_PT_TUNE_TABLE"] = ""
+    else:
+        path = tmp_path / "shapes.txt"
+        path.write_text(table_text)
+        env["QAP_PT_TUNE_TABLE"] = str(path)
+    env.update(extra_env or {})
+    out = subprocess.run([sys.executable, "-c", PROBE.format(tests=os.path.join(ROOT, "tests"))],
+                         capture_output=True, text=True, env=env, timeout=120, check=True).stdout
+    return dict(line.split(" ", 1) for line in out.strip().splitlines())
+
+
+def test_size_rule_without_table(tmp_path):
+    s = _shapes(tmp_path, None)
+    assert s["16"].startswith("64x1,lds32")
+    assert s["65"].startswith("128x1,lds32")
+    assert s["200"].startswith("256x1,")
+    assert s["300"].startswith("256x2,hbm")
+    assert s["700"].startswith("256x4,hbm")
+
+
+def test_table_rows_apply_by_size_flow_and_placement(tmp_path):
+    table = textwrap.dedent("""\
+        # comment line
+        64 dense lds32 64 1   # trailing comment
+        128 dense lds32 64 2
+        256 dense hbm 128 2
+        1024 dense hbm 256 4
+        128 sparse lds32 128 1
+        """)
+    s = _shapes(tmp_path, table)
+    assert s["40"].startswith("64x1,lds32")
+    assert s["100"].startswith("64x2,lds32")      # second dense lds32 row
+    assert s["129"].startswith("256x1,")          # lds32 row ends at 128; size rule
+    assert s["300"].startswith("256x4,hbm")       # 1024-row of the hbm placement
+    assert s["700"].startswith("256x4,hbm")
+
+
+def test_bad_rows_are_ignored_and_shapes_must_cover_n(tmp_path):
+    table = textwrap.dedent("""\
+        64 dense lds32 96 1
+        64 dense lds32
+        garbage
+        128 dense lds32 64 1
+        200 dense lds64 64 4
+        """)
+    s = _shapes(tmp_path, table)
+    assert s["40"].startswith("64x1,lds32")      # 96x1 is not a kernel shape
+    assert s["100"].startswith("128x1,lds32")    # 64x1 cannot hold 100 candidates
+    assert s["200"].startswith("256x1,")         # unknown placement name
+
+
+def test_environment_shape_overrides_table(tmp_path):
+    table = "128 dense lds32 64 2\n"
+    s = _shapes(tmp_path, table, {"QAP_PT_SHAPE": "128,1"})
+    assert s["100"].startswith("128x1,lds32")
+
+
+def test_rates_choose_among_placements_that_fit(tmp_path):
+    # n=64 fits the int32 LDS placement; a table that rates the in-HBM placement higher
+    # for n <= 64 moves it there, while n=100 (no hbm row rated) keeps the default order.
+    table = textwrap.dedent("""\
+        64 dense lds32 64 1 1.0e8
+        64 dense hbm 64 1 2.0e8
+        128 dense lds32 128 1 1.0e8
+        """)
+    s = _shapes(tmp_path, table)
+    assert s["40"].startswith("64x1,hbm")
+    assert s["100"].startswith("128x1,lds32")
+    # an explicit placement request still wins
+    s = _shapes(tmp_path, table, {"QAP_PT_IMODE": "0"})
+    assert s["40"].startswith("64x1,lds32")
