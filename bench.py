@@ -113,16 +113,18 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=float), q)) if len(xs) else None
 
 
-def _chunked_run(pool, sweeps, chunks):
+def _chunked_run(pool, sweeps, chunks, verbose=False):
     """Runs `sweeps` in `chunks` equal pieces; returns the device-run clock after each
     piece (host reads between pieces excluded) and every instance's best cost then."""
     step = sweeps // chunks
     clock, traj = 0.0, []
-    for _ in range(chunks):
+    for c in range(chunks):
         t0 = time.perf_counter()
         pool.run(step)
         clock += time.perf_counter() - t0
         traj.append((clock, pool.best_costs()))
+        if verbose and (c + 1) % 10 == 0:
+            print(f"#   {c + 1}/{chunks} pieces, {clock:.1f} s", flush=True)
     return traj
 
 
@@ -146,7 +148,7 @@ def batch_vs_plain(pkg, args):
              for k, (F, D) in enumerate(I.batch_c3())]
     pool = pkg.Pool(insts)
     print(f"# batched: {len(insts)} instances, {sweeps} sweeps in {chunks} pieces", flush=True)
-    traj = _chunked_run(pool, sweeps, chunks)
+    traj = _chunked_run(pool, sweeps, chunks, verbose=True)
     att_b = sum(pool.result(k)["attempts"] for k in range(len(insts)))
     fm, cm = pool.verify()
     pool.close()

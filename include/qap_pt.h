@@ -130,15 +130,18 @@ int qap_pt_pool_destroy(qap_pt_pool *pool);
 
 /* --- serving loop: continuous batching (no counterpart in the reference, which solves
  *     one instance per process; this is the request loop around the same solve path).
- *     Requests are copied at submit.  A worker thread admits queued requests in arrival
- *     order into pools (up to max_batch per pool, max_pools in flight) whenever their
- *     device footprint fits the budget; admitted pools run concurrently on their own
- *     streams.  A request's result does not depend on what it was batched with. ------ */
+ *     Requests are copied at submit.  A worker thread keeps slot banks resident on the
+ *     device -- pools of fixed-capacity instance slots per size class (n <= 64, 128, 256,
+ *     512, 1024; up to 64 replicas), created when the device budget holds them -- and runs
+ *     every request loaded in a bank in the same kernel launches; between launches it
+ *     releases finished requests and loads queued ones into free slots, so requests that
+ *     arrive while others run join the next launch.  Requests no bank class holds run as
+ *     pools of their own.  A request's result does not depend on what it ran with. ----- */
 typedef struct qap_srv qap_srv;
 
 typedef struct {
-    int32_t max_pools;       /* pools in flight (>= 1), default 4                        */
-    int32_t max_batch;       /* requests merged into one pool (>= 1), default 64          */
+    int32_t max_pools;       /* slot banks + separate pools at once (>= 1), default 4     */
+    int32_t max_batch;       /* slots per bank (>= 1), default 64                          */
     int64_t device_budget;   /* device bytes the server may hold; 0 = free HBM - reserve  */
     int64_t reserve_bytes;   /* kept free when device_budget == 0, default 1 GiB          */
     double batch_window_s;   /* hold a partial batch up to this long for more requests    */

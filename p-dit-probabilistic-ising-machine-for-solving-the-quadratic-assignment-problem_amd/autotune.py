@@ -6,7 +6,7 @@ shape (threads x candidates per thread) with n <= threads*cpl < 2n (at least 64x
 timed on a full-chip load: one instance with enough ladders of 32 replicas to give
 `--replicas` workgroups, one exchange per sweep, HIP-event time of the sweep launches.
 The fastest shape per (bucket, flow, placement) becomes a row
-    n_max  dense|sparse  lds32|lds16|hbm  nt  cpl  rate
+    n_max  dense|sparse  lds32|lds16|hbm|lazy  nt  cpl  rate
 of tune/shapes_gfx950.txt, with every measured rate in a comment line above it.  Each
 placement is forced with QAP_PT_IMODE; up to n=256 every placement is timed on the same
 binary-flow data (which the uint16 placement admits), so the rates compare placements
@@ -26,7 +26,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tune", "shapes_gfx950.txt")
 SHAPES = [(64, 1), (64, 2), (64, 4), (128, 1), (128, 2), (256, 1), (256, 2), (256, 4)]
-PLACEMENTS = {0: "lds32", 1: "lds16", 2: "hbm"}
+PLACEMENTS = {0: "lds32", 1: "lds16", 2: "hbm", 3: "lazy"}
 DEFAULT_SIZES = [32, 48, 64, 96, 128, 160, 192, 256, 384, 512, 768, 1024]
 
 
@@ -97,7 +97,7 @@ def main(argv=None):
     for n in sizes:
         for dense in (True, False):
             flow = "dense" if dense else "sparse"
-            for placement in (0, 1, 2):
+            for placement in (0, 1, 2, 3):
                 rates = []
                 for shape in candidate_shapes(n):
                     rate, got = measure(pkg, n, dense, placement, shape, args.replicas,

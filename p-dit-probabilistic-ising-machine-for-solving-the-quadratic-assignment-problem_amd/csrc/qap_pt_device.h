@@ -116,6 +116,8 @@ struct PoolDev {
     const int32_t *F, *D, *Ft, *Dt;
     const int32_t *Fdiag, *Ddiag;
     const int32_t *fclass;    // per facility: equal id <=> identical flow row and column
+    const int32_t *fspan;     // per facility: nonzero columns of its flow row and column,
+                              //   lo | hi << 10 | count << 20 (empty: lo 1023, hi 0, 0)
     const float *beta2f;      // per ladder position (exp2 units, float)
     const double *c20;        // per ladder position: 20 / beta2 (double)
     int32_t *perm, *pinv, *best_perm, *I;

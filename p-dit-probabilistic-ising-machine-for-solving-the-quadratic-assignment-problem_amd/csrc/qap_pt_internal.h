@@ -5,7 +5,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/qap_pt.h"
+#include "qap_pt_device.h"
 
 namespace qap {
 
@@ -37,5 +40,24 @@ struct TunedShape {
 // process from $QAP_PT_TUNE_TABLE (empty: no table) or else from
 // <directory of libqap_pt.so>/../tune/shapes_gfx950.txt; a missing file means no table.
 const TunedShape *tuned_shape(int n, bool sparse, int im);
+
+// Slot mode (qap_pt_slots.cpp): a pool of `slots` fixed-capacity instance slots that the
+// serving loop loads and releases between launches.  Every call except slots_launch
+// expects the previous launch on the pool stream to have finished.
+int64_t slots_device_bytes(int32_t slots, int32_t n_cap, int32_t r_cap, int32_t t_cap,
+                           int32_t rt_cap);
+int slots_create(int32_t slots, int32_t n_cap, int32_t r_cap, int32_t t_cap, int32_t rt_cap,
+                 qap_pt_pool **out);
+bool slots_fit(const qap_pt_pool *p, const qap_problem &pb, const qap_pt_params &pr);
+int slots_free(const qap_pt_pool *p);    // a free slot, or -1
+int slots_in_use(const qap_pt_pool *p);
+// Enqueues the instance's uploads, force matrices and costs; it joins the next launch.
+int slots_load(qap_pt_pool *p, int slot, const qap_problem &pb, const qap_pt_params &pr);
+// Enqueues `intervals` exchange intervals of every loaded slot and records `done`.
+int slots_launch(qap_pt_pool *p, int intervals, hipEvent_t done);
+// Slots whose run has ended (limit reached, goal, halted); state receives every slot's.
+int slots_finished(qap_pt_pool *p, std::vector<int> &done, std::vector<InstState> &state);
+int slots_halt(qap_pt_pool *p, int slot);
+int slots_release(qap_pt_pool *p, int slot);
 
 }  // namespace qap

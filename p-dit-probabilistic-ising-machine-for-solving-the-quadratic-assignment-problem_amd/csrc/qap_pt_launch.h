@@ -16,6 +16,8 @@ enum IMode : int {
     IM_LDS32 = 0,  // int32 in LDS
     IM_LDS16 = 1,  // uint16 holding -I in LDS (F, D >= 0 and bound <= 65535)
     IM_HBM = 2,    // int32 in HBM, updated in place
+    IM_LAZY = 3,   // int32 in HBM, rank-2 updates held in LDS and applied in batches
+                   // (symmetric instances; 16-bit update vectors)
 };
 
 struct SweepShape {
@@ -27,9 +29,14 @@ struct SweepShape {
 
 bool sweep_shape_supported(int nt, int cpl);
 int sweep_lds_bytes(int n, int ld_s, int im);
+// LDS of the lazy kernel for size n, and whether it holds at least 8 pending updates
+int lazy_lds_bytes(int n);
+bool lazy_fits(int n);
 
 hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,
                         int lds_bytes, hipStream_t s);
+hipError_t launch_sweep_lazy(const PoolDev &P, const SweepShape &sh, const int32_t *list,
+                             int nblocks, int lds_bytes, hipStream_t s);
 hipError_t launch_exchange(const PoolDev &P, int count, hipStream_t s);
 hipError_t launch_set_limits(InstState *state, const int64_t *limits, int count, hipStream_t s);
 hipError_t launch_halt(InstState *state, int count, hipStream_t s);

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <tuple>
@@ -25,6 +26,7 @@
 #include "../../include/qap_pt.h"
 #include "qap_pt_launch.h"
 #include "qap_pt_internal.h"
+#include "qap_pt_pool_impl.h"
 
 using namespace qap;
 
@@ -101,6 +103,8 @@ int round4i(int x) { return (x + 3) & ~3; }
 // full row groups need no per-lane predication.  Both LDS copies have 16-byte rows (an
 // odd number of 4-word quads for int32, of 8-column octets for uint16) so the rank-2
 // update runs on 128-bit LDS accesses.
+}  // namespace
+namespace qap {
 int hbm_stride(int n) { return round4i(n + 1); }
 
 int lds_stride(int n, int im) {
@@ -113,6 +117,8 @@ int lds_stride(int n, int im) {
     if (q8 % 2 == 0) ++q8;
     return 8 * q8;
 }
+}  // namespace qap
+namespace {
 
 // Threads x candidates per thread and force-matrix placement for size n.
 // Placement: QAP_PT_IMODE=0|1|2 when that one fits (LDS budget, uint16 range); else the
@@ -120,23 +126,29 @@ int lds_stride(int n, int im) {
 // the first of lds32 -> lds16 -> hbm that fits.  Shape: QAP_PT_SHAPE="nt,cpl" (tests and
 // the autotuner), else the tuned table's row for (n, flow class, placement), else a size
 // rule.
-SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool sparse, int *ld_s) {
+SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool sparse,
+                        bool small_uv, int *ld_s) {
     SweepShape s{};
     s.sym = sym;
     const bool fit32 = sweep_lds_bytes(n, lds_stride(n, IM_LDS32), IM_LDS32) <= kMaxLdsBytes;
     const bool fit16 = nonneg && force_bound <= 65535 &&
                        sweep_lds_bytes(n, lds_stride(n, IM_LDS16), IM_LDS16) <= kMaxLdsBytes;
+    // lazy: symmetric, update vectors within 16 bits, room for 8 pending updates
+    const bool lazy = sym && small_uv && lazy_fits(n);
     const char *im_env = std::getenv("QAP_PT_IMODE");
     const int want = im_env ? std::atoi(im_env) : -1;
     if (want == IM_HBM) s.im = IM_HBM;
     else if (want == IM_LDS16 && fit16) s.im = IM_LDS16;
+    else if (want == IM_LAZY && lazy) s.im = IM_LAZY;
     else if (fit32) s.im = IM_LDS32;
     else if (fit16) s.im = IM_LDS16;
+    else if (lazy) s.im = IM_LAZY;
     else s.im = IM_HBM;
     if (want < 0) {
         // With tuned rates for every placement that fits, take the fastest (a uint16
         // matrix halves the LDS footprint, so more replicas share a CU).
-        const int cand[3] = {fit32 ? IM_LDS32 : -1, fit16 ? IM_LDS16 : -1, IM_HBM};
+        const int cand[4] = {fit32 ? IM_LDS32 : -1, fit16 ? IM_LDS16 : -1, lazy ? IM_LAZY : -1,
+                             IM_HBM};
         double best_rate = 0.0;
         int best_im = -1;
         bool all_rated = true;
@@ -186,6 +198,15 @@ SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool 
     return s;
 }
 
+const char *placement_name(int im) {
+    switch (im) {
+    case IM_LDS32: return "lds32";
+    case IM_LDS16: return "lds16";
+    case IM_LAZY: return "lazy";
+    default: return "hbm";
+    }
+}
+
 // Facility classes: equal id <=> identical flow column and row, i.e. swapping the two
 // facilities leaves every flow difference v1, v2 of the rank-2 update zero.
 std::vector<int32_t> facility_classes(int n, const int32_t *F) {
@@ -202,6 +223,20 @@ std::vector<int32_t> facility_classes(int n, const int32_t *F) {
         cls[a] = it->second;
     }
     return cls;
+}
+
+// Nonzero columns of facility f's flow row and column, packed lo | hi << 10 | count << 20
+// (n <= 1024).  The flow differences v of a swap of a and b are zero outside
+// [min(lo_a, lo_b), max(hi_a, hi_b)] and have at most count_a + count_b nonzeros.
+int32_t facility_span(int n, const int32_t *F, int f) {
+    int lo = 1023, hi = 0, cnt = 0;
+    for (int c = 0; c < n; ++c)
+        if (F[(int64_t)f * n + c] != 0 || F[(int64_t)c * n + f] != 0) {
+            lo = std::min(lo, c);
+            hi = std::max(hi, c);
+            ++cnt;
+        }
+    return lo | (hi << 10) | (cnt << 20);
 }
 
 double offdiag_density(int n, const int32_t *F) {
@@ -224,56 +259,10 @@ void init_perm(int n, uint64_t seed, uint32_t stream, uint32_t tag, int32_t *per
     }
 }
 
-template <typename T>
-hipError_t dalloc(T **p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    return hipMalloc((void **)p, count * sizeof(T));
-}
 
 }  // namespace
 
-struct Bucket {
-    SweepShape shape;
-    int lds_bytes = 0;
-    std::vector<int32_t> reps;
-    int32_t *d_list = nullptr;
-};
-
-struct qap_pt_pool {
-    int device = 0;
-    int count = 0;
-    hipStream_t stream = nullptr;
-    std::vector<qap_pt_params> params;
-    std::vector<InstDesc> desc;
-    std::vector<SweepShape> shapes;
-    std::vector<Bucket> buckets;
-    std::vector<int64_t> limits;  // mega-iteration limit per instance (host mirror)
-    int total_reps = 0, total_ladders = 0;
-    int64_t total_p = 0, total_I = 0, total_mat = 0, total_vec = 0, total_beta = 0;
-    PoolDev P{};
-    // owned device buffers
-    InstDesc *d_inst = nullptr;
-    InstState *d_state = nullptr;
-    int32_t *d_rep_inst = nullptr, *d_F = nullptr, *d_D = nullptr, *d_Ft = nullptr, *d_Dt = nullptr;
-    int32_t *d_Fdiag = nullptr, *d_Ddiag = nullptr, *d_fclass = nullptr;
-    float *d_beta2f = nullptr;
-    double *d_c20 = nullptr;
-    int32_t *d_perm = nullptr, *d_pinv = nullptr, *d_best_perm = nullptr, *d_I = nullptr;
-    int64_t *d_E = nullptr, *d_bestE = nullptr, *d_limits = nullptr;
-    uint64_t *d_accepted = nullptr, *d_cells = nullptr;
-    int32_t *d_pos = nullptr, *d_order = nullptr, *d_tss = nullptr;
-    int64_t *d_ladder_best = nullptr;
-    std::vector<hipStream_t> side;     // one per extra bucket (default priority)
-    hipEvent_t ev_fork = nullptr;
-    std::vector<hipEvent_t> ev_join;
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    int graph_depth = 0;
-    double seconds = 0.0;
-};
-
-namespace {
+namespace qap {
 
 int validate(const qap_problem &pb, const qap_pt_params &pr, int k) {
     if (pb.n < 1 || pb.n > QAP_PT_MAX_N)
@@ -316,6 +305,13 @@ int validate(const qap_problem &pb, const qap_pt_params &pr, int k) {
     return QAP_PT_OK;
 }
 
+}  // namespace qap
+namespace {
+
+
+}  // namespace
+namespace qap {
+
 void pool_free(qap_pt_pool *p) {
     if (!p) return;
     (void)hipSetDevice(p->device);
@@ -324,7 +320,7 @@ void pool_free(qap_pt_pool *p) {
     if (p->graph) (void)hipGraphDestroy(p->graph);
     for (auto &b : p->buckets)
         if (b.d_list) (void)hipFree(b.d_list);
-    void *bufs[] = {p->d_inst,  p->d_state,     p->d_rep_inst, p->d_F,      p->d_D, p->d_fclass,
+    void *bufs[] = {p->d_inst,  p->d_state,     p->d_rep_inst, p->d_F,      p->d_D, p->d_fclass, p->d_fspan,
                     p->d_Ft,    p->d_Dt,        p->d_Fdiag,    p->d_Ddiag,  p->d_beta2f,
                     p->d_c20,   p->d_perm,      p->d_pinv,     p->d_best_perm, p->d_I,
                     p->d_E,     p->d_bestE,     p->d_limits,   p->d_accepted, p->d_cells, p->d_pos,
@@ -337,6 +333,10 @@ void pool_free(qap_pt_pool *p) {
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
+
+}  // namespace qap
+namespace {
+
 
 // One graph = `depth` mega-iterations: exchange, then every sweep bucket.  Buckets
 // (different kernel instantiations) run concurrently on side streams that fork from
@@ -392,6 +392,139 @@ int build_graph(qap_pt_pool *p, int depth) {
     return QAP_PT_OK;
 }
 
+// Everything one instance needs on the device, built on the host: descriptor (offsets
+// left to the caller), kernel shape, matrices and their transposes, diagonals, facility
+// classes and flow spans, per-rung betas, and the initial permutations of every replica
+// (CPU_PIM.cpp:109-162).  Ladder positions start as the identity (order == pos).
+}  // namespace
+namespace qap {
+
+void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, HostInstance &h) {
+    const int n = pb.n;
+    const int64_t nn = (int64_t)n * n;
+    InstDesc &d = h.d;
+    const bool sym = is_sym_zero_diag(n, pb.F) && is_sym_zero_diag(n, pb.D);
+    bool nonneg = true;
+    for (int64_t x = 0; x < nn && nonneg; ++x) nonneg = pb.F[x] >= 0 && pb.D[x] >= 0;
+    const Bounds bnd = compute_bounds(n, pb.F, pb.D);
+    int ld_s = 0;
+    const bool sparse = offdiag_density(n, pb.F) < kSparseFlow;
+    // lazy updates hold u (|u| <= 2 max|D|) and v (|v| <= 4 max|F|) in 16 bits
+    const bool small_uv = 2 * bnd.maxD <= 32767 && 4 * bnd.maxF <= 32767;
+    h.shape = choose_shape(n, sym, nonneg, bnd.force, sparse, small_uv, &ld_s);
+    d = InstDesc{};
+    d.n = n;
+    d.ld = hbm_stride(n);
+    d.ld_s = ld_s;
+    d.sparse_upd = sparse ? 1 : 0;
+    // u = distance differences (<= 2 max|D|), v = flow differences (<= 4 max|F|)
+    d.mul24 = (2 * bnd.maxD < (1 << 23) && 4 * bnd.maxF < (1 << 23)) ? 1 : 0;
+    d.R = pr.R;
+    d.T = pr.T;
+    d.spe = pr.sweeps_per_exchange;
+    d.kick_after = pr.kick_after;
+    d.has_goal = pr.has_goal ? 1 : 0;
+    d.sym = sym ? 1 : 0;
+    d.goal = pr.goal;
+    d.seed_lo = (uint32_t)pr.seed;
+    d.seed_hi = (uint32_t)(pr.seed >> 32);
+
+    h.F.assign(pb.F, pb.F + nn);
+    h.D.assign(pb.D, pb.D + nn);
+    h.Ft.resize(nn);
+    h.Dt.resize(nn);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            h.Ft[(int64_t)i * n + j] = pb.F[(int64_t)j * n + i];
+            h.Dt[(int64_t)i * n + j] = pb.D[(int64_t)j * n + i];
+        }
+    h.Fd.resize(n);
+    h.Dd.resize(n);
+    h.span.resize(n);
+    for (int i = 0; i < n; ++i) {
+        h.Fd[i] = pb.F[(int64_t)i * n + i];
+        h.Dd[i] = pb.D[(int64_t)i * n + i];
+        h.span[i] = facility_span(n, pb.F, i);
+    }
+    h.cls = facility_classes(n, pb.F);
+    for (int l = 0; l < pr.R; ++l) {
+        const double b2 = pr.beta_ln[l] * kLog2e;
+        h.beta2f.push_back((float)b2);
+        h.c20.push_back(20.0 / b2);
+    }
+    std::vector<int32_t> pi(n), inv(n);
+    for (int t = 0; t < pr.T; ++t) {
+        if (!(pr.flags & QAP_PT_INIT_PER_REPLICA)) {
+            if (pr.init_perm) std::copy(pr.init_perm, pr.init_perm + n, pi.begin());
+            else init_perm(n, pr.seed, (uint32_t)t, 1u, pi.data());
+        }
+        for (int l = 0; l < pr.R; ++l) {
+            if (pr.flags & QAP_PT_INIT_PER_REPLICA) {
+                if (pr.init_perm) std::copy(pr.init_perm, pr.init_perm + n, pi.begin());
+                else init_perm(n, pr.seed, (uint32_t)(t * pr.R + l), 2u, pi.data());
+            }
+            h.perm.insert(h.perm.end(), pi.begin(), pi.end());
+            for (int i = 0; i < n; ++i) inv[pi[i]] = i;
+            h.pinv.insert(h.pinv.end(), inv.begin(), inv.end());
+            h.pos.push_back(l);
+        }
+    }
+}
+
+void order_bucket(const qap_pt_pool *p, Bucket &B, const std::function<int(int32_t)> &inst_of) {
+    std::map<int, std::vector<int32_t>> by_inst;
+    for (int32_t g : B.reps) by_inst[inst_of(g)].push_back(g);
+    std::vector<int> insts;
+    for (auto &kv : by_inst) insts.push_back(kv.first);
+    std::stable_sort(insts.begin(), insts.end(),
+                     [&](int x, int y) { return p->desc[x].n > p->desc[y].n; });
+    std::vector<int32_t> order_list;
+    for (size_t g0 = 0; g0 < insts.size(); g0 += 8) {
+        const size_t g1 = std::min(insts.size(), g0 + 8);
+        size_t longest = 0;
+        for (size_t x = g0; x < g1; ++x) longest = std::max(longest, by_inst[insts[x]].size());
+        for (size_t r = 0; r < longest; ++r)
+            for (size_t x = g0; x < g1; ++x) {
+                const auto &v = by_inst[insts[x]];
+                if (r < v.size()) order_list.push_back(v[r]);
+            }
+    }
+    B.reps.swap(order_list);
+}
+
+void bind_pool(qap_pt_pool *p) {
+    PoolDev &P = p->P;
+    P.inst = p->d_inst;
+    P.state = p->d_state;
+    P.rep_inst = p->d_rep_inst;
+    P.F = p->d_F;
+    P.D = p->d_D;
+    P.Ft = p->d_Ft;
+    P.Dt = p->d_Dt;
+    P.Fdiag = p->d_Fdiag;
+    P.Ddiag = p->d_Ddiag;
+    P.fclass = p->d_fclass;
+    P.fspan = p->d_fspan;
+    P.beta2f = p->d_beta2f;
+    P.c20 = p->d_c20;
+    P.perm = p->d_perm;
+    P.pinv = p->d_pinv;
+    P.best_perm = p->d_best_perm;
+    P.I = p->d_I;
+    P.E = p->d_E;
+    P.bestE = p->d_bestE;
+    P.accepted = p->d_accepted;
+    P.cells = p->d_cells;
+    P.pos = p->d_pos;
+    P.order = p->d_order;
+    P.tss = p->d_tss;
+    P.ladder_best = p->d_ladder_best;
+}
+
+}  // namespace qap
+namespace {
+
+
 int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_params *params,
                      qap_pt_pool **out) {
     if (!out) return fail(QAP_PT_E_INVAL, "out is NULL");
@@ -413,36 +546,18 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     p->shapes.resize(count);
     p->limits.assign(count, 0);
 
-    std::vector<int32_t> hF, hD, hFt, hDt, hFd, hDd, hCls, rep_inst, perm, pinv, pos, order;
+    std::vector<int32_t> hF, hD, hFt, hDt, hFd, hDd, hCls, hSpan, rep_inst, perm, pinv, pos, order;
     std::vector<float> beta2f;
     std::vector<double> c20;
     std::map<std::tuple<int, int, int, int>, int> bucket_of;
+    auto append = [](auto &dst, const auto &src) { dst.insert(dst.end(), src.begin(), src.end()); };
     for (int k = 0; k < count; ++k) {
-        const qap_problem &pb = problems[k];
-        const qap_pt_params &pr = params[k];
-        const int n = pb.n;
+        HostInstance h;
+        prepare_instance(problems[k], params[k], h);
         InstDesc &d = p->desc[k];
-        const bool sym = is_sym_zero_diag(n, pb.F) && is_sym_zero_diag(n, pb.D);
-        bool nonneg = true;
-        for (int64_t x = 0; x < (int64_t)n * n && nonneg; ++x) nonneg = pb.F[x] >= 0 && pb.D[x] >= 0;
-        const Bounds bnd = compute_bounds(n, pb.F, pb.D);
-        int ld_s = 0;
-        const bool sparse = offdiag_density(n, pb.F) < kSparseFlow;
-        const SweepShape sh = choose_shape(n, sym, nonneg, bnd.force, sparse, &ld_s);
-        p->shapes[k] = sh;
-        d.n = n;
-        d.ld = hbm_stride(n);
-        d.ld_s = ld_s;
-        d.sparse_upd = sparse ? 1 : 0;
-        // u = distance differences (<= 2 max|D|), v = flow differences (<= 4 max|F|)
-        d.mul24 = (2 * bnd.maxD < (1 << 23) && 4 * bnd.maxF < (1 << 23)) ? 1 : 0;
-        d.R = pr.R;
-        d.T = pr.T;
-        d.spe = pr.sweeps_per_exchange;
-        d.kick_after = pr.kick_after;
-        d.has_goal = pr.has_goal ? 1 : 0;
-        d.sym = sym ? 1 : 0;
-        d.goal = pr.goal;
+        d = h.d;
+        p->shapes[k] = h.shape;
+        const int n = d.n, RT = d.R * d.T;
         d.mat_off = p->total_mat;
         d.vec_off = p->total_vec;
         d.p_base = p->total_p;
@@ -450,55 +565,29 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
         d.rep_base = p->total_reps;
         d.ladder_base = p->total_ladders;
         d.beta_off = (int32_t)p->total_beta;
-        d.seed_lo = (uint32_t)pr.seed;
-        d.seed_hi = (uint32_t)(pr.seed >> 32);
-        const int RT = pr.R * pr.T;
         p->total_mat += (int64_t)n * n;
         p->total_vec += n;
         p->total_p += (int64_t)RT * n;
         p->total_I += (int64_t)RT * n * d.ld;
         p->total_reps += RT;
-        p->total_ladders += pr.T;
-        p->total_beta += pr.R;
-
-        hF.insert(hF.end(), pb.F, pb.F + (int64_t)n * n);
-        hD.insert(hD.end(), pb.D, pb.D + (int64_t)n * n);
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j < n; ++j) {
-                hFt.push_back(pb.F[(int64_t)j * n + i]);
-                hDt.push_back(pb.D[(int64_t)j * n + i]);
-            }
-        for (int i = 0; i < n; ++i) {
-            hFd.push_back(pb.F[(int64_t)i * n + i]);
-            hDd.push_back(pb.D[(int64_t)i * n + i]);
-        }
-        const std::vector<int32_t> cls = facility_classes(n, pb.F);
-        hCls.insert(hCls.end(), cls.begin(), cls.end());
-        for (int l = 0; l < pr.R; ++l) {
-            const double b2 = pr.beta_ln[l] * kLog2e;
-            beta2f.push_back((float)b2);
-            c20.push_back(20.0 / b2);
-        }
-        std::vector<int32_t> pi(n);
-        for (int t = 0; t < pr.T; ++t) {
-            if (!(pr.flags & QAP_PT_INIT_PER_REPLICA)) {
-                if (pr.init_perm) std::copy(pr.init_perm, pr.init_perm + n, pi.begin());
-                else init_perm(n, pr.seed, (uint32_t)t, 1u, pi.data());
-            }
-            for (int l = 0; l < pr.R; ++l) {
-                if (pr.flags & QAP_PT_INIT_PER_REPLICA) {
-                    if (pr.init_perm) std::copy(pr.init_perm, pr.init_perm + n, pi.begin());
-                    else init_perm(n, pr.seed, (uint32_t)(t * pr.R + l), 2u, pi.data());
-                }
-                perm.insert(perm.end(), pi.begin(), pi.end());
-                std::vector<int32_t> inv(n);
-                for (int i = 0; i < n; ++i) inv[pi[i]] = i;
-                pinv.insert(pinv.end(), inv.begin(), inv.end());
-                pos.push_back(l);
-                order.push_back(l);
-                rep_inst.push_back(k);
-            }
-        }
+        p->total_ladders += d.T;
+        p->total_beta += d.R;
+        append(hF, h.F);
+        append(hD, h.D);
+        append(hFt, h.Ft);
+        append(hDt, h.Dt);
+        append(hFd, h.Fd);
+        append(hDd, h.Dd);
+        append(hCls, h.cls);
+        append(hSpan, h.span);
+        append(beta2f, h.beta2f);
+        append(c20, h.c20);
+        append(perm, h.perm);
+        append(pinv, h.pinv);
+        append(pos, h.pos);
+        append(order, h.pos);
+        for (int r = 0; r < RT; ++r) rep_inst.push_back(k);
+        const SweepShape &sh = h.shape;
         const auto key = std::make_tuple(sh.nt, sh.cpl, sh.im, (int)sh.sym);
         auto it = bucket_of.find(key);
         int bi;
@@ -515,31 +604,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
         for (int r = 0; r < RT; ++r) B.reps.push_back(d.rep_base + r);
     }
 
-    // Order every bucket's workgroups: larger n first (longest sweeps start first),
-    // and groups of 8 instances interleaved replica by replica so that consecutive
-    // workgroup ids (dispatched round-robin over the 8 XCDs) keep each instance on one
-    // XCD, where its F and D stay in that XCD's L2.
-    for (auto &B : p->buckets) {
-        std::map<int, std::vector<int32_t>> by_inst;
-        for (int32_t g : B.reps) by_inst[rep_inst[g]].push_back(g);
-        std::vector<int> insts;
-        for (auto &kv : by_inst) insts.push_back(kv.first);
-        std::stable_sort(insts.begin(), insts.end(), [&](int x, int y) {
-            return p->desc[x].n > p->desc[y].n;
-        });
-        std::vector<int32_t> order_list;
-        for (size_t g0 = 0; g0 < insts.size(); g0 += 8) {
-            const size_t g1 = std::min(insts.size(), g0 + 8);
-            size_t longest = 0;
-            for (size_t x = g0; x < g1; ++x) longest = std::max(longest, by_inst[insts[x]].size());
-            for (size_t r = 0; r < longest; ++r)
-                for (size_t x = g0; x < g1; ++x) {
-                    const auto &v = by_inst[insts[x]];
-                    if (r < v.size()) order_list.push_back(v[r]);
-                }
-        }
-        B.reps.swap(order_list);
-    }
+    for (auto &B : p->buckets) order_bucket(p, B, [&](int32_t g) { return rep_inst[g]; });
 
     auto bail = [&](int rc) {
         pool_free(p);
@@ -566,6 +631,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(dalloc(&p->d_Fdiag, p->total_vec));
     POOL_TRY(dalloc(&p->d_Ddiag, p->total_vec));
     POOL_TRY(dalloc(&p->d_fclass, p->total_vec));
+    POOL_TRY(dalloc(&p->d_fspan, p->total_vec));
     POOL_TRY(dalloc(&p->d_beta2f, p->total_beta));
     POOL_TRY(dalloc(&p->d_c20, p->total_beta));
     POOL_TRY(dalloc(&p->d_perm, p->total_p));
@@ -607,6 +673,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(up(p->d_Fdiag, hFd.data(), 4 * hFd.size()));
     POOL_TRY(up(p->d_Ddiag, hDd.data(), 4 * hDd.size()));
     POOL_TRY(up(p->d_fclass, hCls.data(), 4 * hCls.size()));
+    POOL_TRY(up(p->d_fspan, hSpan.data(), 4 * hSpan.size()));
     POOL_TRY(up(p->d_beta2f, beta2f.data(), 4 * beta2f.size()));
     POOL_TRY(up(p->d_c20, c20.data(), 8 * c20.size()));
     POOL_TRY(up(p->d_perm, perm.data(), 4 * perm.size()));
@@ -621,31 +688,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(up(p->d_ladder_best, ladder_best.data(), 8 * ladder_best.size()));
     POOL_TRY(hipMemset(p->d_I, 0, sizeof(int32_t) * std::max<int64_t>(p->total_I, 1)));
 
-    PoolDev &P = p->P;
-    P.inst = p->d_inst;
-    P.state = p->d_state;
-    P.rep_inst = p->d_rep_inst;
-    P.F = p->d_F;
-    P.D = p->d_D;
-    P.Ft = p->d_Ft;
-    P.Dt = p->d_Dt;
-    P.Fdiag = p->d_Fdiag;
-    P.Ddiag = p->d_Ddiag;
-    P.fclass = p->d_fclass;
-    P.beta2f = p->d_beta2f;
-    P.c20 = p->d_c20;
-    P.perm = p->d_perm;
-    P.pinv = p->d_pinv;
-    P.best_perm = p->d_best_perm;
-    P.I = p->d_I;
-    P.E = p->d_E;
-    P.bestE = p->d_bestE;
-    P.accepted = p->d_accepted;
-    P.cells = p->d_cells;
-    P.pos = p->d_pos;
-    P.order = p->d_order;
-    P.tss = p->d_tss;
-    P.ladder_best = p->d_ladder_best;
+    bind_pool(p);
 
     // Force matrices (CPU_PIM.cpp:164-181) and costs (:184-192) on the device.
     for (int k = 0; k < count; ++k) {
@@ -874,7 +917,7 @@ int64_t pool_device_bytes(int32_t count, const qap_problem *problems, const qap_
     for (int k = 0; k < count; ++k) {
         const int64_t n = problems[k].n, RT = (int64_t)params[k].R * params[k].T;
         const int64_t ld = hbm_stride((int)n);
-        bytes += 4 * (4 * n * n + 3 * n);            // F, D, Ft, Dt, diagonals, classes
+        bytes += 4 * (4 * n * n + 4 * n);            // F, D, Ft, Dt, diagonals, classes, spans
         bytes += 4 * RT * n * ld;                     // force matrices
         bytes += 4 * 3 * RT * n;                      // perm, pinv, best_perm
         bytes += RT * (8 * 4 + 4 * 3);                // E, bestE, accepted, cells, pos, order, rep_inst
@@ -1037,7 +1080,7 @@ int qap_pt_pool_shape(qap_pt_pool *p, int32_t inst, char *buf, int32_t buflen) {
     const SweepShape &s = p->shapes[inst];
     const InstDesc &d = p->desc[inst];
     std::snprintf(buf, buflen, "%dx%d,%s,%s,%s", s.nt, s.cpl,
-                  s.im == IM_LDS32 ? "lds32" : (s.im == IM_LDS16 ? "lds16" : "hbm"),
+                  placement_name(s.im),
                   s.sym ? "sym" : "general", d.sparse_upd ? "sparse-flow" : "dense-flow");
     return QAP_PT_OK;
 }
@@ -1045,9 +1088,9 @@ int qap_pt_pool_shape(qap_pt_pool *p, int32_t inst, char *buf, int32_t buflen) {
 int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen) {
     if (n < 1 || n > QAP_PT_MAX_N || !buf || buflen < 1) return fail(QAP_PT_E_INVAL, "bad argument");
     int ld_s = 0;
-    const SweepShape s = choose_shape(n, true, true, (int64_t)1 << 40, false, &ld_s);
+    const SweepShape s = choose_shape(n, true, true, (int64_t)1 << 40, false, true, &ld_s);
     std::snprintf(buf, buflen, "%dx%d,%s", s.nt, s.cpl,
-                  s.im == IM_LDS32 ? "lds32" : (s.im == IM_LDS16 ? "lds16" : "hbm"));
+                  placement_name(s.im));
     return QAP_PT_OK;
 }
 

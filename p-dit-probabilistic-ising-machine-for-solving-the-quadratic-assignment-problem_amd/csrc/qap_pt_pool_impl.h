@@ -1,0 +1,107 @@
+// qap_pt_pool_impl.h -- the pool's host-side structure, shared by the batch entry points
+// (qap_pt_pool.cpp) and the slot mode the serving loop runs on (qap_pt_slots.cpp).
+// Private to the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <vector>
+
+#include "../../include/qap_pt.h"
+#include "qap_pt_launch.h"
+
+namespace qap {
+
+// Everything one instance needs on the device, built on the host (prepare_instance).
+struct HostInstance {
+    InstDesc d{};
+    SweepShape shape{};
+    std::vector<int32_t> F, D, Ft, Dt, Fd, Dd, cls, span, perm, pinv, pos;
+    std::vector<float> beta2f;
+    std::vector<double> c20;
+};
+
+struct Bucket {
+    SweepShape shape;
+    int lds_bytes = 0;
+    std::vector<int32_t> reps;
+    int32_t *d_list = nullptr;
+};
+
+}  // namespace qap
+
+struct qap_pt_pool {
+    int device = 0;
+    int count = 0;
+    hipStream_t stream = nullptr;
+    std::vector<qap_pt_params> params;
+    std::vector<qap::InstDesc> desc;
+    std::vector<qap::SweepShape> shapes;
+    std::vector<qap::Bucket> buckets;
+    std::vector<int64_t> limits;  // mega-iteration limit per instance (host mirror)
+    int total_reps = 0, total_ladders = 0;
+    int64_t total_p = 0, total_I = 0, total_mat = 0, total_vec = 0, total_beta = 0;
+    qap::PoolDev P{};
+    // owned device buffers
+    qap::InstDesc *d_inst = nullptr;
+    qap::InstState *d_state = nullptr;
+    int32_t *d_rep_inst = nullptr, *d_F = nullptr, *d_D = nullptr, *d_Ft = nullptr, *d_Dt = nullptr;
+    int32_t *d_Fdiag = nullptr, *d_Ddiag = nullptr, *d_fclass = nullptr, *d_fspan = nullptr;
+    float *d_beta2f = nullptr;
+    double *d_c20 = nullptr;
+    int32_t *d_perm = nullptr, *d_pinv = nullptr, *d_best_perm = nullptr, *d_I = nullptr;
+    int64_t *d_E = nullptr, *d_bestE = nullptr, *d_limits = nullptr;
+    uint64_t *d_accepted = nullptr, *d_cells = nullptr;
+    int32_t *d_pos = nullptr, *d_order = nullptr, *d_tss = nullptr;
+    int64_t *d_ladder_best = nullptr;
+    std::vector<hipStream_t> side;     // one per extra bucket (default priority)
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_depth = 0;
+    double seconds = 0.0;
+    // slot mode: `count` fixed-capacity slots that instances are loaded into and released
+    // from while the pool runs (qap_pt_slots.cpp)
+    bool slotted = false;
+    int n_cap = 0, r_cap = 0, t_cap = 0, rt_cap = 0;
+    std::vector<uint8_t> used;
+    std::vector<qap::HostInstance> staged;  // host sources of a slot's pending uploads
+    std::vector<qap::InstState> st_load;         // per slot: the state uploaded by its load
+    std::vector<std::vector<int64_t>> fill64;  // per slot: INT64_MAX / zero upload sources
+    bool lists_dirty = false;
+};
+
+
+namespace qap {
+
+#define QAP_HIP_TRY(expr)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return ::qap::fail(e_ == hipErrorOutOfMemory ? QAP_PT_E_NOMEM : QAP_PT_E_DEVICE,    \
+                               "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                               __LINE__);                                                       \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void **)p, count * sizeof(T));
+}
+
+int validate(const qap_problem &pb, const qap_pt_params &pr, int k);
+void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, HostInstance &h);
+// Row stride of a force matrix in HBM for size n.
+int hbm_stride(int n);
+// Orders a bucket's workgroups: larger n first, groups of 8 instances interleaved
+// replica by replica (consecutive workgroup ids go round-robin over the 8 XCDs, so each
+// instance stays on one XCD and its F and D stay in that XCD's L2).
+void order_bucket(const qap_pt_pool *p, Bucket &B, const std::function<int(int32_t)> &inst_of);
+// Fills the device-pointer table P from the pool's buffers.
+void bind_pool(qap_pt_pool *p);
+void pool_free(qap_pt_pool *p);
+
+}  // namespace qap

@@ -1,14 +1,19 @@
 // qap_pt_server.cpp -- serving loop: continuous batching of QAP solve requests on one GPU.
 //
 // Requests are copied at submit time and queued.  A worker thread (the only thread that
-// calls HIP) admits queued requests in arrival order into pools -- up to max_batch
-// requests per pool, up to max_pools pools in flight -- as long as their device
-// footprint fits the budget (explicit, or free HBM minus a reserve).  Each pool runs on
-// its own stream with its whole sweep budget enqueued as graph launches, so pools
-// admitted while others are running overlap on the device.  Completion is detected with
-// an event query; results stay until polled.  A request's trajectory depends only on
-// its own inputs and seed (Philox counters are keyed by the request's seed and replica
-// index, never by slot or pool), so its result is the same alone and in any mix.
+// calls HIP) keeps slot banks resident on the device: pools of fixed-capacity instance
+// slots, one bank per size class (n <= 64, 128, 256, 512, 1024; up to 32 or 64
+// replicas), created when a request of that class arrives and the device budget
+// (explicit, or free HBM minus a reserve) holds the bank.  Each bank runs its loaded
+// requests together, a few exchange intervals per launch (about 8 ms of work) on its own
+// stream; between launches finished requests are released and queued requests are
+// loaded into free slots, so requests that arrive while others run join the very next
+// launch and share every kernel with them.  A request that fits no bank class, or finds
+// no bank room while the budget still holds a pool of its own, runs as a separate pool.
+// A request's trajectory depends only on its own inputs and seed (Philox counters are
+// keyed by the request's seed and replica index, never by slot or bank), so its result
+// is the same alone and in any mix.  Banks idle for a second are freed.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -42,6 +47,24 @@ struct Flight {
     int64_t bytes = 0;
 };
 
+struct Bank {
+    qap_pt_pool *pool = nullptr;
+    int n_cap = 0, rt_cap = 0;
+    hipEvent_t done = nullptr;
+    bool inflight = false;
+    bool fresh = false;                // a launch ended since the state was last read
+    int intervals = 4;                 // per launch, adapted to ~8 ms of device work
+    Clock::time_point launched, idle_since;
+    std::vector<uint64_t> ticket;      // per slot, 0 = free
+    std::vector<Clock::time_point> started;
+    std::vector<uint8_t> halting;      // per slot: timed out, halt requested
+    int64_t bytes = 0;
+};
+
+constexpr int kBankClasses[] = {64, 128, 256, 512, 1024};
+constexpr double kLaunchTarget = 0.008;  // seconds of device work per bank launch
+constexpr double kBankIdleFree = 1.0;    // seconds an empty bank is kept
+
 }  // namespace
 
 struct qap_srv {
@@ -51,6 +74,7 @@ struct qap_srv {
     std::deque<uint64_t> pending;
     std::unordered_map<uint64_t, Request> reqs;
     std::vector<Flight> flights;
+    std::vector<Bank> banks;
     uint64_t next_ticket = 1;
     int64_t bytes_in_use = 0;
     int64_t n_done = 0;
@@ -97,58 +121,225 @@ int64_t available_bytes(qap_srv *s) {
     return (int64_t)free_b - s->cfg.reserve_bytes;
 }
 
-// Admit queued requests (FIFO) while pools and memory allow.  Lock held.
-void admit(qap_srv *s) {
-    while (!s->pending.empty() && (int)s->flights.size() < s->cfg.max_pools) {
-        if (s->cfg.batch_window_s > 0 && (int)s->pending.size() < s->cfg.max_batch) {
-            const Request &oldest = s->reqs[s->pending.front()];
-            if (std::chrono::duration<double>(Clock::now() - oldest.submitted).count() <
-                s->cfg.batch_window_s)
-                return;
+void finish(qap_srv *s, Request &r, int status) {
+    r.status = status;
+    r.res.seconds = std::chrono::duration<double>(Clock::now() - r.submitted).count();
+    ++s->n_done;
+}
+
+// Bank class of a request: the smallest size class that holds n, and 32 or 64 replicas;
+// n_cap 0 when it fits no class.
+void bank_class(const Request &r, int &n_cap, int &rt_cap) {
+    n_cap = 0;
+    rt_cap = 0;
+    const int64_t RT = (int64_t)r.par.R * r.par.T;
+    if (RT > 64) return;
+    rt_cap = RT <= 32 ? 32 : 64;
+    for (int c : kBankClasses)
+        if (r.n <= c) {
+            n_cap = c;
+            return;
         }
-        int64_t avail = available_bytes(s);
-        std::vector<uint64_t> take;
-        int64_t bytes = 0;
-        for (uint64_t t : s->pending) {
-            if ((int)take.size() >= s->cfg.max_batch) break;
-            const int64_t b = s->reqs[t].bytes;
-            if (bytes + b > avail) break;
-            take.push_back(t);
-            bytes += b;
+}
+
+// Launches that ended: finished requests leave their slots with their results; requests
+// past their timeout are halted (they finish after the next launch).  Lock held.
+void service_banks(qap_srv *s) {
+    const auto now = Clock::now();
+    std::vector<int> done;
+    std::vector<qap::InstState> st;
+    for (Bank &b : s->banks) {
+        if (b.inflight) {
+            const hipError_t q = hipEventQuery(b.done);
+            if (q == hipErrorNotReady) continue;
+            b.inflight = false;
+            b.fresh = true;
+            const double t = std::chrono::duration<double>(now - b.launched).count();
+            if (q == hipSuccess && t > 0)  // keep about kLaunchTarget of work per launch
+                b.intervals = std::max(1, std::min(1024, (int)(b.intervals * kLaunchTarget / t + 0.5)));
         }
-        if (take.empty()) {
-            if (s->flights.empty()) {  // nothing will free memory: this request cannot run
-                Request &r = s->reqs[s->pending.front()];
-                r.status = QAP_PT_E_NOMEM;
-                r.res.seconds = std::chrono::duration<double>(Clock::now() - r.submitted).count();
-                s->pending.pop_front();
-                continue;
+        if (!b.fresh) continue;
+        b.fresh = false;
+        int rc = qap::slots_finished(b.pool, done, st);
+        for (int slot = 0; slot < (int)b.ticket.size() && rc; ++slot)
+            if (b.ticket[slot]) done.push_back(slot);  // device error: fail every request
+        for (int slot : done) {
+            Request &r = s->reqs[b.ticket[slot]];
+            r.res = qap_pt_result{};
+            r.res.best_perm = r.best_perm.data();
+            const int rr = rc ? rc : qap::pool_result_internal(b.pool, slot, &r.res);
+            r.res.best_perm = nullptr;
+            finish(s, r, rr ? rr : QAP_SRV_DONE);
+            (void)qap::slots_release(b.pool, slot);
+            b.ticket[slot] = 0;
+            b.halting[slot] = 0;
+        }
+        for (int slot = 0; slot < (int)b.ticket.size() && !rc; ++slot) {
+            if (!b.ticket[slot] || b.halting[slot]) continue;
+            const Request &r = s->reqs[b.ticket[slot]];
+            if (r.par.timeout_s > 0 &&
+                std::chrono::duration<double>(now - b.started[slot]).count() > r.par.timeout_s) {
+                (void)qap::slots_halt(b.pool, slot);
+                b.halting[slot] = 1;
             }
-            return;  // wait for a running pool to finish
         }
-        for (size_t k = 0; k < take.size(); ++k) s->pending.pop_front();
-        std::vector<qap_problem> probs;
-        std::vector<qap_pt_params> pars;
-        for (uint64_t t : take) {
-            probs.push_back(problem_of(s->reqs[t]));
-            pars.push_back(s->reqs[t].par);
+    }
+    for (Bank &b : s->banks) {
+        if (b.inflight || qap::slots_in_use(b.pool) > 0) {
+            b.idle_since = Clock::time_point{};
+        } else if (b.idle_since == Clock::time_point{}) {
+            b.idle_since = now;
         }
-        Flight fl;
-        fl.tickets = take;
-        fl.bytes = bytes;
-        int rc = qap::pool_create_internal((int32_t)take.size(), probs.data(), pars.data(), &fl.pool);
-        if (rc == QAP_PT_OK && hipEventCreateWithFlags(&fl.done, hipEventDisableTiming) != hipSuccess)
-            rc = QAP_PT_E_DEVICE;
-        if (rc == QAP_PT_OK) rc = qap::pool_launch_async(fl.pool, fl.done);
-        if (rc != QAP_PT_OK) {
-            if (fl.done) (void)hipEventDestroy(fl.done);
-            if (fl.pool) qap::pool_destroy_internal(fl.pool);
-            for (uint64_t t : take) s->reqs[t].status = rc;
+    }
+    // empty banks idle for a while give their memory back
+    for (size_t k = 0; k < s->banks.size();) {
+        Bank &b = s->banks[k];
+        if (!b.inflight && b.idle_since != Clock::time_point{} &&
+            (s->stop || std::chrono::duration<double>(now - b.idle_since).count() > kBankIdleFree)) {
+            (void)hipEventDestroy(b.done);
+            qap::pool_destroy_internal(b.pool);
+            s->bytes_in_use -= b.bytes;
+            s->banks.erase(s->banks.begin() + (long)k);
+        } else {
+            ++k;
+        }
+    }
+}
+
+// A bank with a free slot for this class that is between launches, creating one when
+// the budget holds it.  nullptr: wait (or use a pool of its own).  Lock held.
+Bank *bank_for(qap_srv *s, int n_cap, int rt_cap, int &err) {
+    err = QAP_PT_OK;
+    bool busy_room = false;
+    for (Bank &b : s->banks) {
+        if (b.n_cap != n_cap || b.rt_cap != rt_cap) continue;
+        if (qap::slots_free(b.pool) < 0) continue;
+        if (!b.inflight) return &b;
+        busy_room = true;
+    }
+    if (busy_room) return nullptr;  // room opens at the end of the current launch
+    if ((int)(s->banks.size() + s->flights.size()) >= s->cfg.max_pools) return nullptr;
+    const int64_t per_slot = qap::slots_device_bytes(1, n_cap, rt_cap, rt_cap, rt_cap);
+    const int64_t avail = available_bytes(s);
+    // a bank takes at most a quarter of what is available, and at least one slot
+    int slots = (int)std::min<int64_t>(s->cfg.max_batch, (avail / 4) / std::max<int64_t>(per_slot, 1));
+    if (slots < 1) {
+        if (per_slot > avail) return nullptr;
+        slots = 1;
+    }
+    Bank b;
+    b.n_cap = n_cap;
+    b.rt_cap = rt_cap;
+    b.bytes = qap::slots_device_bytes(slots, n_cap, rt_cap, rt_cap, rt_cap);
+    err = qap::slots_create(slots, n_cap, rt_cap, rt_cap, rt_cap, &b.pool);
+    if (err == QAP_PT_OK && hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess)
+        err = qap::fail(QAP_PT_E_DEVICE, "event creation failed");
+    if (err != QAP_PT_OK) {
+        if (b.pool) qap::pool_destroy_internal(b.pool);
+        return nullptr;
+    }
+    b.ticket.assign(slots, 0);
+    b.started.assign(slots, Clock::time_point{});
+    b.halting.assign(slots, 0);
+    s->bytes_in_use += b.bytes;
+    s->banks.push_back(std::move(b));
+    return &s->banks.back();
+}
+
+// A request in a pool of its own (sizes no bank class holds).  Lock held.
+int start_flight(qap_srv *s, uint64_t t) {
+    Request &r = s->reqs[t];
+    Flight fl;
+    fl.tickets = {t};
+    fl.bytes = r.bytes;
+    qap_problem pb = problem_of(r);
+    int rc = qap::pool_create_internal(1, &pb, &r.par, &fl.pool);
+    if (rc == QAP_PT_OK && hipEventCreateWithFlags(&fl.done, hipEventDisableTiming) != hipSuccess)
+        rc = qap::fail(QAP_PT_E_DEVICE, "event creation failed");
+    if (rc == QAP_PT_OK) rc = qap::pool_launch_async(fl.pool, fl.done);
+    if (rc != QAP_PT_OK) {
+        if (fl.done) (void)hipEventDestroy(fl.done);
+        if (fl.pool) qap::pool_destroy_internal(fl.pool);
+        return rc;
+    }
+    r.status = QAP_SRV_RUNNING;
+    s->bytes_in_use += fl.bytes;
+    s->flights.push_back(fl);
+    return QAP_PT_OK;
+}
+
+// Admit queued requests in arrival order while slots and memory allow; a request that
+// has to wait does not hold back later ones of other classes.  Lock held.
+void admit(qap_srv *s) {
+    if (s->pending.empty()) return;
+    if (s->cfg.batch_window_s > 0 && (int)s->pending.size() < s->cfg.max_batch) {
+        const Request &oldest = s->reqs[s->pending.front()];
+        if (std::chrono::duration<double>(Clock::now() - oldest.submitted).count() <
+            s->cfg.batch_window_s)
+            return;
+    }
+    const auto now = Clock::now();
+    for (auto it = s->pending.begin(); it != s->pending.end();) {
+        const uint64_t t = *it;
+        Request &r = s->reqs[t];
+        int n_cap = 0, rt_cap = 0, err = QAP_PT_OK;
+        bank_class(r, n_cap, rt_cap);
+        Bank *b = n_cap ? bank_for(s, n_cap, rt_cap, err) : nullptr;
+        if (b) {
+            const int slot = qap::slots_free(b->pool);
+            const qap_problem pb = problem_of(r);
+            const int rc = qap::slots_load(b->pool, slot, pb, r.par);
+            if (rc) {
+                finish(s, r, rc);
+            } else {
+                b->ticket[slot] = t;
+                b->started[slot] = now;
+                r.status = QAP_SRV_RUNNING;
+            }
+            it = s->pending.erase(it);
             continue;
         }
-        for (uint64_t t : take) s->reqs[t].status = QAP_SRV_RUNNING;
-        s->bytes_in_use += bytes;
-        s->flights.push_back(fl);
+        if (err != QAP_PT_OK && err != QAP_PT_E_NOMEM) {  // the device refused a bank
+            finish(s, r, err);
+            it = s->pending.erase(it);
+            continue;
+        }
+        // no bank room: a pool of its own when the budget holds it (sizes no bank class
+        // holds, or memory too short for a bank)
+        const bool alone = n_cap == 0 || qap::slots_device_bytes(1, n_cap, rt_cap, rt_cap, rt_cap) >
+                                             available_bytes(s);
+        if (alone && (int)(s->banks.size() + s->flights.size()) < s->cfg.max_pools &&
+            r.bytes <= available_bytes(s)) {
+            const int rc = start_flight(s, t);
+            if (rc) finish(s, r, rc);
+            it = s->pending.erase(it);
+            continue;
+        }
+        if (alone && s->banks.empty() && s->flights.empty()) {  // nothing will free memory
+            finish(s, r, QAP_PT_E_NOMEM);
+            it = s->pending.erase(it);
+            continue;
+        }
+        ++it;
+    }
+}
+
+// Banks between launches with loaded requests start their next launch.  Lock held.
+void launch_banks(qap_srv *s) {
+    for (Bank &b : s->banks) {
+        if (b.inflight || qap::slots_in_use(b.pool) == 0) continue;
+        const int rc = qap::slots_launch(b.pool, b.intervals, b.done);
+        if (rc) {  // fail the bank's requests
+            for (int slot = 0; slot < (int)b.ticket.size(); ++slot)
+                if (b.ticket[slot]) {
+                    finish(s, s->reqs[b.ticket[slot]], rc);
+                    (void)qap::slots_release(b.pool, slot);
+                    b.ticket[slot] = 0;
+                }
+            continue;
+        }
+        b.inflight = true;
+        b.launched = Clock::now();
     }
 }
 
@@ -158,13 +349,17 @@ void worker_main(qap_srv *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         if (s->stop) {
-            for (uint64_t t : s->pending) s->reqs[t].status = QAP_PT_E_BUSY;
+            for (uint64_t t : s->pending) finish(s, s->reqs[t], QAP_PT_E_BUSY);
             s->pending.clear();
         }
         collect(s);
-        if (s->stop && s->flights.empty()) break;
+        service_banks(s);
+        if (s->stop && s->flights.empty() && s->banks.empty()) break;
         admit(s);
-        s->cv.wait_for(lk, std::chrono::microseconds(s->flights.empty() ? 2000 : 200));
+        launch_banks(s);
+        bool busy = !s->flights.empty();
+        for (const Bank &b : s->banks) busy = busy || b.inflight;
+        s->cv.wait_for(lk, std::chrono::microseconds(busy ? 100 : 2000));
     }
 }
 
@@ -289,6 +484,7 @@ int qap_srv_stats(qap_srv *s, int64_t *queued, int64_t *running, int64_t *done,
     std::lock_guard<std::mutex> lk(s->mu);
     int64_t run = 0;
     for (auto &f : s->flights) run += (int64_t)f.tickets.size();
+    for (auto &b : s->banks) run += qap::slots_in_use(b.pool);
     if (queued) *queued = (int64_t)s->pending.size();
     if (running) *running = run;
     if (done) *done = s->n_done;

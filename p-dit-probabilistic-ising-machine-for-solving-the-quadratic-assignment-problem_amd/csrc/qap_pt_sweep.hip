@@ -25,69 +25,17 @@
 // ladder of an instance (CPU_PIM.cpp:293-357), staged through LDS.
 #include "qap_pt_device.h"
 #include "qap_pt_launch.h"
+#include "qap_pt_sweep_common.h"
 
 namespace qap {
 
-constexpr int kWave = 64;
-
-// ---------------------------------------------------------------------------
-// Wave64 inclusive scans with DPP.  update_dpp(old, src, ctrl, row_mask, bank_mask,
-// bound_ctrl=false) returns `old` for lanes whose source lane is outside the row or
-// whose row is masked off, so `old` is the identity of the operator.
-// ---------------------------------------------------------------------------
-template <int CTRL, int RMASK>
-__device__ __forceinline__ int dpp_mov(int v, int identity) {
-    return __builtin_amdgcn_update_dpp(identity, v, CTRL, RMASK, 0xf, false);
-}
-
-__device__ __forceinline__ int wave_incl_max(int v) {
-    v = max(v, dpp_mov<0x111, 0xf>(v, INT32_MIN));  // row_shr:1
-    v = max(v, dpp_mov<0x112, 0xf>(v, INT32_MIN));  // row_shr:2
-    v = max(v, dpp_mov<0x114, 0xf>(v, INT32_MIN));  // row_shr:4
-    v = max(v, dpp_mov<0x118, 0xf>(v, INT32_MIN));  // row_shr:8
-    v = max(v, dpp_mov<0x142, 0xa>(v, INT32_MIN));  // row_bcast:15 -> rows 1,3
-    v = max(v, dpp_mov<0x143, 0xc>(v, INT32_MIN));  // row_bcast:31 -> rows 2,3
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t u) {
-    int v = (int)u;
-    v += dpp_mov<0x111, 0xf>(v, 0);
-    v += dpp_mov<0x112, 0xf>(v, 0);
-    v += dpp_mov<0x114, 0xf>(v, 0);
-    v += dpp_mov<0x118, 0xf>(v, 0);
-    v += dpp_mov<0x142, 0xa>(v, 0);
-    v += dpp_mov<0x143, 0xc>(v, 0);
-    return (uint32_t)v;
-}
-
-__device__ __forceinline__ int lane63(int v) { return __builtin_amdgcn_readlane(v, kWave - 1); }
-
-__host__ __device__ constexpr int round4(int x) { return (x + 3) & ~3; }
-__host__ __device__ constexpr int round8(int x) { return (x + 7) & ~7; }
-
-// Workgroup barrier that orders LDS only.  __syncthreads() also waits for every
-// outstanding global load (vmcnt(0)), which would stall the row prefetches; the sweep's
-// barriers only publish LDS data, so they fence the LDS address space alone.  With a
-// single wave no s_barrier is needed; the fence keeps the compiler's order.
-template <int NW>
-__device__ __forceinline__ void lds_sync() {
-    if constexpr (NW == 1) {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront", "local");
-    } else {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-        __builtin_amdgcn_s_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    }
-}
-
 // red_s (ints): [0,16) max partials, [16,80) sum partials [q*16 + wave],
 // [96] winner b, [97] winner delta, [98] winner's partner location j,
-// [100,116) / [116,132) nonzero-column min / max partials.
+// [140, 140 + 2 NW) per-wave rewritten-cell counts (uint64) at the end of a launch.
 constexpr int kRedWords = 160;
 // p, pinv, cls, u1, u2, col, v1, v2, rnd, ddiag, drow x2, dcol x2, frow x2, ftrow x2,
-// vd1, vd2 -- each round8(n) words
-constexpr int kSmallArrays = 20;
+// vd1, vd2, fspan -- each round8(n) words
+constexpr int kSmallArrays = 21;
 
 __host__ __device__ inline int sweep_lds_words(int n, int ld_s, int im) {
     const int small = kSmallArrays * round8(n) + kRedWords;
@@ -160,7 +108,7 @@ __device__ __forceinline__ void dense_update(IT *const colp[CPL], int32_t rs, in
 
 // Rank-2 update over the column range of vectors [V0, V1) of an LDS force matrix with
 // 16-byte rows: W = 4 int32 or W = 8 uint16 (holding -I) columns per 128-bit access.
-// Each lane owns one vector column and a fixed residue of rows; four rows in flight.
+// Each lane owns one vector column and a fixed residue of rows.
 // vd1/vd2 hold v for every column (zero outside the nonzero set).
 template <int NT, int IM, bool SYM, bool M24>
 __device__ __forceinline__ void vec_update(int4 *I4, int ldv, int n, int V0, int V1,
@@ -185,8 +133,55 @@ __device__ __forceinline__ void vec_update(int4 *I4, int ldv, int n, int V0, int
             a2[t4] = a2[t4 + 1] = a2[t4 + 2] = a2[t4 + 3] = 0;
         }
     }
-    constexpr int G = 4;
-    for (int k0 = ry; k0 < n; k0 += G * rpp) {
+    // Rows go in groups of G loads in flight, with no bounds test inside a full group;
+    // the last, partial group clamps its loads and predicates its stores.
+    constexpr int G = 8;
+    const int step = rpp * ldv;  // int4 elements between consecutive rows of this lane
+    // uint16 placement: the word of columns (2h, 2h+1) gets -u v for both halves at once
+    // with packed 16-bit integer multiply-adds (modulo 2^16, exact: the entries fit)
+    typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+    u16x2 pk1[4], pk2[4];
+    if constexpr (IM == IM_LDS16) {
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            pk1[h] = u16x2{(unsigned short)a1[2 * h], (unsigned short)a1[2 * h + 1]};
+            pk2[h] = u16x2{(unsigned short)a2[2 * h], (unsigned short)a2[2 * h + 1]};
+        }
+    }
+    auto update_row = [&](int4 x, int32_t w1, int32_t w2) -> int4 {
+        int32_t e[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            if constexpr (IM == IM_LDS16) {
+                // two uint16 (-I) per word: subtract the increments, wrap to 16 bits
+                const unsigned short n1 = (unsigned short)(-w1), n2 = (unsigned short)(-w2);
+                u16x2 r = __builtin_bit_cast(u16x2, e[h]);
+                r = u16x2{n1, n1} * pk1[h] + r;
+                if constexpr (!SYM) r = u16x2{n2, n2} * pk2[h] + r;
+                e[h] = __builtin_bit_cast(int32_t, r);
+            } else {
+                int32_t dv = umul<M24>(w1, a1[h]);
+                if constexpr (!SYM) dv += umul<M24>(w2, a2[h]);
+                e[h] += dv;
+            }
+        }
+        return int4{e[0], e[1], e[2], e[3]};
+    };
+    int4 *rowp = I4 + ry * ldv + vi;
+    int k0 = ry;
+    for (; k0 + (G - 1) * rpp < n; k0 += G * rpp, rowp += G * step) {
+        int4 x[G];
+        int32_t w1[G], w2[G];
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            x[r] = rowp[r * step];
+            w1[r] = u1_s[k0 + r * rpp];
+            w2[r] = SYM ? 0 : u2_s[k0 + r * rpp];
+        }
+#pragma unroll
+        for (int r = 0; r < G; ++r) rowp[r * step] = update_row(x[r], w1[r], w2[r]);
+    }
+    if (k0 < n) {  // fewer than G rows left for this lane
         int4 x[G];
         int32_t w1[G], w2[G];
 #pragma unroll
@@ -197,32 +192,8 @@ __device__ __forceinline__ void vec_update(int4 *I4, int ldv, int n, int V0, int
             w2[r] = SYM ? 0 : u2_s[k];
         }
 #pragma unroll
-        for (int r = 0; r < G; ++r) {
-            const int k = k0 + r * rpp;
-            if (k < n) {
-                int32_t e[4] = {x[r].x, x[r].y, x[r].z, x[r].w};
-#pragma unroll
-                for (int h = 0; h < 4; ++h) {
-                    if constexpr (IM == IM_LDS16) {
-                        // two uint16 (-I) per word: subtract the increments, wrap to 16 bits
-                        const int t = 2 * h;
-                        int32_t d0 = umul<M24>(w1[r], a1[t]), d1 = umul<M24>(w1[r], a1[t + 1]);
-                        if constexpr (!SYM) {
-                            d0 += umul<M24>(w2[r], a2[t]);
-                            d1 += umul<M24>(w2[r], a2[t + 1]);
-                        }
-                        const uint32_t lo = ((uint32_t)e[h] - (uint32_t)d0) & 0xffffu;
-                        const uint32_t hi = (((uint32_t)e[h] >> 16) - (uint32_t)d1) & 0xffffu;
-                        e[h] = (int32_t)(lo | (hi << 16));
-                    } else {
-                        int32_t dv = umul<M24>(w1[r], a1[h]);
-                        if constexpr (!SYM) dv += umul<M24>(w2[r], a2[h]);
-                        e[h] += dv;
-                    }
-                }
-                I4[k * ldv + vi] = int4{e[0], e[1], e[2], e[3]};
-            }
-        }
+        for (int r = 0; r < G; ++r)
+            if (k0 + r * rpp < n) rowp[r * step] = update_row(x[r], w1[r], w2[r]);
     }
 }
 
@@ -265,7 +236,8 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     int32_t *ftrow_s = frow_s + 2 * n4;                          // 2 buffers
     int32_t *vd1_s = ftrow_s + 2 * n4;                           // v of every column
     int32_t *vd2_s = vd1_s + n4;
-    int32_t *red_s = vd2_s + n4;
+    int32_t *fsp_s = vd2_s + n4;                                 // nonzero flow columns
+    int32_t *red_s = fsp_s + n4;
     IT *I_s = reinterpret_cast<IT *>(red_s + kRedWords);
 
     const int32_t *__restrict__ F = P.F + d.mat_off;
@@ -294,6 +266,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         p_s[x] = pg[x];
         pinv_s[x] = pinvg[x];
         cls_s[x] = P.fclass[d.vec_off + x];
+        fsp_s[x] = P.fspan[d.vec_off + x];
         if constexpr (!SYM) ddiag_s[x] = Ddg[x];
     }
     // HBM rows are 16-byte aligned (ld % 4 == 0): a (row, 4-column chunk) tiling of the
@@ -306,12 +279,13 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     const int cx_st = tid & (cw_st - 1), ry_st = tid >> sh_st;
     constexpr int kStageU = 8;
     if constexpr (IM == IM_LDS32) {  // 16-byte rows on both sides: straight quad copies
-        const int4 *Ig4 = reinterpret_cast<const int4 *>(Ig);
-        int4 *Is4 = reinterpret_cast<int4 *>(I_s);
+        // (a native vector type: HIP's int4 struct copies are not promoted to registers)
+        const v4i *Ig4 = reinterpret_cast<const v4i *>(Ig);
+        v4i *Is4 = reinterpret_cast<v4i *>(I_s);
         const int ld4 = ld_s >> 2;
         for (int cx = cx_st; cx < L4; cx += cw_st) {
             for (int k0 = ry_st; k0 < n; k0 += rw_st * kStageU) {
-                int4 v[kStageU];
+                v4i v[kStageU];
 #pragma unroll
                 for (int u = 0; u < kStageU; ++u) v[u] = Ig4[(int64_t)min(k0 + u * rw_st, n - 1) * L4 + cx];
 #pragma unroll
@@ -358,7 +332,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     int64_t E = P.E[g];
     int64_t bestE = P.bestE[g];
     uint64_t acc = P.accepted[g];
-    uint64_t cells = P.cells[g];
+    uint64_t cells = 0;  // this wave's share: n x the nonzero flow differences it owns
     const int pos = P.pos[g];
     const float beta2f = P.beta2f[d.beta_off + pos];
     const double c20 = P.c20[d.beta_off + pos];
@@ -581,6 +555,14 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                 E -= dsel;
                 ++acc;
                 if (cls_s[a] != cls_s[b]) {
+                    // The flow differences are nonzero only inside the union of the two
+                    // facilities' nonzero flow columns (static, from the host).  A dense
+                    // instance updates that range with 128-bit accesses; a sparse one counts
+                    // the nonzeros first and compacts them when the range is much longer.
+                    const uint32_t sa = (uint32_t)fsp_s[a], sb = (uint32_t)fsp_s[b];
+                    const int s_lo = (int)min(sa & 1023u, sb & 1023u);
+                    const int s_hi = (int)max((sa >> 10) & 1023u, (sb >> 10) & 1023u);
+                    constexpr int VW = (IM == IM_LDS16) ? 8 : 4;
                     int32_t v1[CPL], v2[CPL];
                     uint32_t nz = 0;
 #pragma unroll
@@ -610,8 +592,30 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                         }
                     }
                     uint32_t nnz = (uint32_t)n;
-                    int c_lo = 0, c_hi = n - 1;
-                    if constexpr (IM != IM_HBM) {  // v of every column for the vector path
+                    uint32_t base = 0;
+                    if (sparse_upd) {  // exact count of the nonzero columns
+                        const uint32_t inc = wave_incl_sum(nz);
+                        if constexpr (NW == 1) {
+                            base = inc - nz;
+                            nnz = (uint32_t)lane63((int)inc);
+                        } else {
+                            if (lane == kWave - 1) red_s[16 + wid] = (int)inc;
+                            lds_sync<NW>();
+                            uint32_t before = 0, tot = 0;
+#pragma unroll
+                            for (int ww = 0; ww < NW; ++ww) {
+                                const uint32_t x = (uint32_t)red_s[16 + ww];
+                                before += (ww < wid) ? x : 0u;
+                                tot += x;
+                            }
+                            base = before + inc - nz;
+                            nnz = tot;
+                        }
+                    }
+                    const int c_lo = s_lo, c_hi = s_hi;
+                    const bool vec_path =
+                        IM != IM_HBM && (!sparse_upd || c_hi - c_lo + 1 <= 4 * (int)nnz + 4 * VW);
+                    if (vec_path) {  // v of every column for the vector path
 #pragma unroll
                         for (int q = 0; q < CPL; ++q) {
                             const int c = tid + q * NT;
@@ -620,45 +624,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                                 if constexpr (!SYM) vd2_s[c] = v2[q];
                             }
                         }
-                    }
-                    if (sparse_upd) {
-                        // count, range and compaction of the nonzero columns
-                        int lo = n, hi = -1;
-#pragma unroll
-                        for (int q = 0; q < CPL; ++q)
-                            if ((v1[q] | v2[q]) != 0) {
-                                lo = min(lo, tid + q * NT);
-                                hi = max(hi, tid + q * NT);
-                            }
-                        const uint32_t inc = wave_incl_sum(nz);
-                        const int wlo = -wave_incl_max(-lo), whi = wave_incl_max(hi);
-                        uint32_t base;
-                        if constexpr (NW == 1) {
-                            base = inc - nz;
-                            nnz = (uint32_t)lane63((int)inc);
-                            c_lo = lane63(wlo);
-                            c_hi = lane63(whi);
-                        } else {
-                            if (lane == kWave - 1) {
-                                red_s[16 + wid] = (int)inc;
-                                red_s[100 + wid] = wlo;
-                                red_s[116 + wid] = whi;
-                            }
-                            lds_sync<NW>();
-                            uint32_t before = 0, tot = 0;
-                            c_lo = n;
-                            c_hi = -1;
-#pragma unroll
-                            for (int ww = 0; ww < NW; ++ww) {
-                                const uint32_t x = (uint32_t)red_s[16 + ww];
-                                before += (ww < wid) ? x : 0u;
-                                tot += x;
-                                c_lo = min(c_lo, red_s[100 + ww]);
-                                c_hi = max(c_hi, red_s[116 + ww]);
-                            }
-                            base = before + inc - nz;
-                            nnz = tot;
-                        }
+                    } else if (sparse_upd) {  // compaction of the nonzero columns
 #pragma unroll
                         for (int q = 0; q < CPL; ++q) {
                             if ((v1[q] | v2[q]) != 0) {
@@ -670,13 +636,14 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                         }
                     }
                     lds_sync<NW>();
-                    constexpr int VW = (IM == IM_LDS16) ? 8 : 4;
-                    const bool vec_path =
-                        IM != IM_HBM && (!sparse_upd || c_hi - c_lo + 1 <= 4 * (int)nnz + 4 * VW);
-                    cells += (uint64_t)n * (vec_path ? (uint32_t)(c_hi - c_lo + 1) : nnz);
-                    if (nnz == 0) {
-                        // flow differences all zero: nothing to update
-                    } else if (vec_path) {
+                    {
+                        uint32_t cnt = 0;
+#pragma unroll
+                        for (int q = 0; q < CPL; ++q)
+                            cnt += (uint32_t)__popcll(__ballot((v1[q] | v2[q]) != 0));
+                        cells += (uint64_t)n * cnt;
+                    }
+                    if (vec_path) {
                         int4 *I4 = reinterpret_cast<int4 *>(I_s);
                         const int V0 = c_lo / VW, V1 = c_hi / VW + 1;
                         if (d.mul24)
@@ -749,6 +716,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         }
     }
 
+    if (lane == 0) reinterpret_cast<uint64_t *>(red_s + 140)[wid] = cells;
     lds_sync<NW>();
     for (int x = tid; x < n; x += NT) pg[x] = p_s[x];
 #pragma unroll
@@ -757,8 +725,8 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         if (bb < n) pinvg[bb] = pv[q];
     }
     if constexpr (IM == IM_LDS32) {
-        int4 *Ig4 = reinterpret_cast<int4 *>(Ig);
-        const int4 *Is4 = reinterpret_cast<const int4 *>(I_s);
+        v4i *Ig4 = reinterpret_cast<v4i *>(Ig);
+        const v4i *Is4 = reinterpret_cast<const v4i *>(I_s);
         const int ld4 = ld_s >> 2;
         for (int cx = cx_st; cx < L4; cx += cw_st)
             for (int k = ry_st; k < n; k += rw_st) Ig4[(int64_t)k * L4 + cx] = Is4[k * ld4 + cx];
@@ -782,7 +750,13 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         P.E[g] = E;
         P.bestE[g] = bestE;
         P.accepted[g] = acc;
-        P.cells[g] = cells;
+        uint64_t tot = cells;
+        if constexpr (NW > 1) {
+            const uint64_t *wc = reinterpret_cast<const uint64_t *>(red_s + 140);
+#pragma unroll
+            for (int w = 1; w < NW; ++w) tot += wc[w];
+        }
+        P.cells[g] += tot;
     }
 }
 
@@ -930,6 +904,7 @@ static hipError_t launch_sweep_c(const PoolDev &P, const SweepShape &sh, const i
 hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,
                         int lds_bytes, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
+    if (sh.im == IM_LAZY) return launch_sweep_lazy(P, sh, list, nblocks, lds_bytes, s);
     switch (sh.nt * 8 + sh.cpl) {
     case 64 * 8 + 1: return launch_sweep_c<64, 1>(P, sh, list, nblocks, lds_bytes, s);
     case 64 * 8 + 2: return launch_sweep_c<64, 2>(P, sh, list, nblocks, lds_bytes, s);
@@ -951,7 +926,9 @@ bool sweep_shape_supported(int nt, int cpl) {
     }
 }
 
-int sweep_lds_bytes(int n, int ld_s, int im) { return 4 * sweep_lds_words(n, ld_s, im); }
+int sweep_lds_bytes(int n, int ld_s, int im) {
+    return im == IM_LAZY ? lazy_lds_bytes(n) : 4 * sweep_lds_words(n, ld_s, im);
+}
 
 hipError_t launch_exchange(const PoolDev &P, int count, hipStream_t s) {
     hipLaunchKernelGGL(exchange_kernel, dim3(count), dim3(64), 0, s, P);

@@ -1,0 +1,427 @@
+// qap_pt_sweep_lazy.hip -- p-dit sweeps of symmetric instances whose force matrix stays in
+// HBM (n too large for LDS), with the rank-2 updates accumulated lazily.
+//
+// The force matrix in HBM holds I_base; the rank-2 updates of the last K accepted moves,
+// I += u_k v_k^T (u = distance differences, v = flow differences, CPU_PIM.cpp:263-269),
+// wait in LDS as 16-bit vectors stored k-contiguous per row: U[x][k], V[c][k].  Every
+// entry the update rule reads (CPU_PIM.cpp:220-232) is I_base + sum_k U[x][k] V[c][k],
+// formed as 16-bit dot products:
+//   row i      I[i][b] for lane b:  rowI[b] + dot(U[i], V[b])         (rowI = I_base row i)
+//   column a   I[j][a] for lane b:  colA[j] + dot(U[j], V[a])         (colA[x] = I_base[x][a])
+//   diagonal   dg[c] = I[pinv(c)][c], kept exact move by move; I[i][a] = dg[a].
+// When K reaches the capacity the LDS holds (kmax <= 32) the pending updates are applied to
+// I_base in one pass over the matrix, so kmax accepted moves cost one read and one write of
+// I instead of kmax; every launch ends with such a flush, so I in HBM is exact between
+// launches.  All sums are 32-bit two's complement: the entries themselves fit (the host
+// checks the force bound), so wrapped partial sums still give the exact entries.
+//
+// One workgroup per replica as in qap_pt_sweep.hip (candidate b owned by thread
+// b % NT); the heat-bath pick, Philox counters and exchange semantics are the same, so
+// trajectories are identical to the in-LDS and in-HBM kernels.
+#include "qap_pt_device.h"
+#include "qap_pt_launch.h"
+#include "qap_pt_sweep_common.h"
+
+namespace qap {
+
+namespace {
+
+constexpr int kLazyRed = 160;  // reduction slots (same use as the other sweep kernel)
+constexpr int kLazyMaxK = 32;
+// p, pinv, cls, rnd, dg, drow x2, frow x2, rowI x2, colA x2 -- round8(n) words each
+constexpr int kLazyArrays = 13;
+
+__host__ __device__ inline int lazy_fixed_words(int n) { return kLazyArrays * round8(n) + kLazyRed; }
+
+// pending-update capacity a launch with `lds_bytes` of LDS gives an instance of size n
+// (a multiple of 8: U and V rows are read 8 entries at a time)
+__host__ __device__ inline int lazy_kmax(int n, int lds_bytes) {
+    const int avail = lds_bytes - 4 * lazy_fixed_words(n);
+    int k = avail / (4 * round8(n));  // U and V: 2 x 2 bytes per row entry
+    k = k < kLazyMaxK ? k : kLazyMaxK;
+    return k & ~7;
+}
+
+using v4h = int __attribute__((ext_vector_type(4)));  // 8 packed int16
+
+// sum_k x[k] y[k] over packed signed 16-bit entries, 8 per 128-bit word: each half is
+// sign-extended and multiplied in 32-bit integer arithmetic (exact)
+__device__ __forceinline__ int32_t dot8(v4h x, v4h y, int32_t acc) {
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const int32_t xl = (int32_t)(int16_t)(x[h] & 0xffff), xh = x[h] >> 16;
+        const int32_t yl = (int32_t)(int16_t)(y[h] & 0xffff), yh = y[h] >> 16;
+        acc += xl * yl + xh * yh;
+    }
+    return acc;
+}
+
+}  // namespace
+
+int lazy_lds_bytes(int n) {
+    const int want = 4 * lazy_fixed_words(n) + 4 * round8(n) * kLazyMaxK;
+    return want < kMaxLdsBytes ? want : kMaxLdsBytes;
+}
+
+bool lazy_fits(int n) { return lazy_kmax(n, lazy_lds_bytes(n)) >= 8; }
+
+template <int NT, int CPL>
+__global__ __launch_bounds__(NT) void sweep_lazy_kernel(PoolDev P, const int32_t *__restrict__ rep_list,
+                                                        int lds_bytes) {
+    constexpr int NW = NT / kWave;
+    extern __shared__ __align__(16) int32_t smem[];
+
+    const int g = rep_list[blockIdx.x];
+    const int inst = P.rep_inst[g];
+    if (!P.state[inst].active) return;
+    const InstDesc d = P.inst[inst];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int n = d.n, ld = d.ld, n4 = round8(n);
+    const int local = g - d.rep_base;
+    const int kmax = lazy_kmax(n, lds_bytes);
+    const int kpad = kmax;  // U / V row length in int16 (a multiple of 8)
+    if (kmax < 8) return;   // the host places an instance here only when 8 updates fit
+
+    int32_t *p_s = smem;
+    int32_t *pinv_s = p_s + n4;
+    int32_t *cls_s = pinv_s + n4;
+    uint32_t *rnd_s = reinterpret_cast<uint32_t *>(cls_s + n4);
+    int32_t *dg_s = reinterpret_cast<int32_t *>(rnd_s + n4);
+    int32_t *drow_s = dg_s + n4;      // 2 buffers
+    int32_t *frow_s = drow_s + 2 * n4;  // 2 buffers
+    int32_t *rowI_s = frow_s + 2 * n4;  // 2 buffers
+    int32_t *colA_s = rowI_s + 2 * n4;  // 2 buffers
+    int32_t *red_s = colA_s + 2 * n4;
+    int16_t *U_s = reinterpret_cast<int16_t *>(red_s + kLazyRed);  // [n4][kpad]
+    int16_t *V_s = U_s + (size_t)n4 * kpad;                         // [n4][kpad]
+
+    const int32_t *__restrict__ F = P.F + d.mat_off;
+    const int32_t *__restrict__ D = P.D + d.mat_off;
+    int32_t *pg = P.perm + d.p_base + (int64_t)local * n;
+    int32_t *pinvg = P.pinv + d.p_base + (int64_t)local * n;
+    int32_t *bestpg = P.best_perm + d.p_base + (int64_t)local * n;
+    int32_t *Ig = P.I + d.I_base + (int64_t)local * n * ld;
+
+    auto zero_pending = [&]() {
+        v4h *u4 = reinterpret_cast<v4h *>(U_s);
+        const int words = n4 * kpad / 8 * 2;  // U and V are contiguous
+        for (int x = tid; x < words; x += NT) u4[x] = v4h{0, 0, 0, 0};
+    };
+
+    // ---- stage the replica --------------------------------------------------
+    for (int x = tid; x < n; x += NT) {
+        p_s[x] = pg[x];
+        pinv_s[x] = pinvg[x];
+        cls_s[x] = P.fclass[d.vec_off + x];
+    }
+    zero_pending();
+    lds_sync<NW>();
+    int a_cur = p_s[0];
+    for (int x = tid; x < n; x += NT) {
+        dg_s[x] = Ig[(int64_t)pinv_s[x] * ld + x];
+        drow_s[x] = D[x];
+        frow_s[x] = F[(int64_t)a_cur * n + x];
+        rowI_s[x] = Ig[x];
+        colA_s[x] = Ig[(int64_t)x * ld + a_cur];
+    }
+    lds_sync<NW>();
+
+    int64_t E = P.E[g];
+    int64_t bestE = P.bestE[g];
+    uint64_t acc = P.accepted[g];
+    uint64_t cells = 0;
+    const int pos = P.pos[g];
+    const float beta2f = P.beta2f[d.beta_off + pos];
+    const double c20 = P.c20[d.beta_off + pos];
+    const uint32_t sweep0 = P.state[inst].sweep_base;
+
+    int32_t pv[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+        const int b = tid + q * NT;
+        pv[q] = (b < n) ? pinv_s[b] : 0;
+    }
+    int buf = 0, K = 0, a_reg = a_cur;
+    bool stale = false;  // a flush made the prefetched I_base row / column out of date
+
+    // I_base += U V^T over the whole matrix, then no pending update is left
+    auto flush = [&]() {
+        if (K == 0) return;
+        const int groups = K / 8 + ((K & 7) != 0);
+        v4h vr[CPL][kLazyMaxK / 8];
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            const int c = min(tid + q * NT, n - 1);
+#pragma unroll
+            for (int gg = 0; gg < kLazyMaxK / 8; ++gg)
+                vr[q][gg] = gg < groups ? reinterpret_cast<const v4h *>(V_s + (size_t)c * kpad)[gg]
+                                        : v4h{0, 0, 0, 0};
+        }
+        constexpr int G = 4;
+        for (int x0 = 0; x0 < n; x0 += G) {
+            int32_t cur[G][CPL];
+#pragma unroll
+            for (int r = 0; r < G; ++r)
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const int x = min(x0 + r, n - 1), c = min(tid + q * NT, n - 1);
+                    cur[r][q] = Ig[(int64_t)x * ld + c];
+                }
+#pragma unroll
+            for (int r = 0; r < G; ++r) {
+                const int x = x0 + r;
+                if (x >= n) break;
+                const v4h *ux = reinterpret_cast<const v4h *>(U_s + (size_t)x * kpad);
+                int32_t sum[CPL];
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) sum[q] = cur[r][q];
+#pragma unroll
+                for (int gg = 0; gg < kLazyMaxK / 8; ++gg) {
+                    if (gg >= groups) break;
+                    const v4h uu = ux[gg];
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q) sum[q] = dot8(uu, vr[q][gg], sum[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const int c = tid + q * NT;
+                    if (c < n) Ig[(int64_t)x * ld + c] = sum[q];
+                }
+            }
+        }
+        __syncthreads();  // the matrix writes reach every wave before it is read again
+        zero_pending();
+        K = 0;
+        stale = true;
+        lds_sync<NW>();
+    };
+
+    for (int s = 0; s < d.spe; ++s) {
+        const uint32_t sweep = sweep0 + (uint32_t)s;
+        lds_sync<NW>();
+        for (int x = tid; x < n; x += NT)
+            rnd_s[x] = philox4x32_10(u32x4{(uint32_t)x, sweep, (uint32_t)local, 0u}, d.seed_lo,
+                                     d.seed_hi)
+                           .x;
+        lds_sync<NW>();
+
+        for (int i = 0; i < n; ++i) {
+            const int a = a_reg;
+            const uint32_t ri = rnd_s[i];
+            if (a != a_cur || stale) {  // the prefetched flow row / I_base data is not for this site
+                a_cur = a;
+                for (int x = tid; x < n; x += NT) {
+                    frow_s[buf * n4 + x] = F[(int64_t)a * n + x];
+                    rowI_s[buf * n4 + x] = Ig[(int64_t)i * ld + x];
+                    colA_s[buf * n4 + x] = Ig[(int64_t)x * ld + a];
+                }
+                stale = false;
+                __syncthreads();
+            }
+            const int32_t *drow = drow_s + buf * n4;
+            const int32_t *frow = frow_s + buf * n4;  // F[a][.]
+            const int32_t *rowI = rowI_s + buf * n4;  // I_base[i][.]
+            const int32_t *colA = colA_s + buf * n4;  // I_base[.][a]
+            const int32_t Iia = dg_s[a];
+            const int groups = K / 8 + ((K & 7) != 0);
+            const v4h *Ui = reinterpret_cast<const v4h *>(U_s + (size_t)i * kpad);
+            const v4h *Va = reinterpret_cast<const v4h *>(V_s + (size_t)a * kpad);
+
+            // (1) deltas of every candidate from I_base plus the pending updates
+            int32_t dE[CPL], rv[CPL], cv[CPL];
+            int lmax = 0;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const int b = tid + q * NT;
+                dE[q] = INT32_MIN;
+                rv[q] = 0;
+                cv[q] = 0;
+                if (b < n) {
+                    const int j = pv[q];
+                    int32_t r = rowI[b], c = colA[j];
+                    const v4h *Vb = reinterpret_cast<const v4h *>(V_s + (size_t)b * kpad);
+                    const v4h *Uj = reinterpret_cast<const v4h *>(U_s + (size_t)j * kpad);
+                    for (int gg = 0; gg < groups; ++gg) {
+                        r = dot8(Ui[gg], Vb[gg], r);
+                        c = dot8(Uj[gg], Va[gg], c);
+                    }
+                    rv[q] = r;  // I[i][b]
+                    cv[q] = c;  // I[j][a]
+                    const int32_t v = r + c - Iia - dg_s[b] - 4 * drow[j] * frow[b];
+                    dE[q] = v;
+                    lmax = max(lmax, v);
+                }
+            }
+            // prefetch for the next site: distance row i+1, flow row p(i+1), I_base row
+            // i+1 and I_base column p(i+1); stored at the end of this update
+            const int inext = (i + 1 == n) ? 0 : i + 1;
+            const int a_nx = p_s[inext];
+            int32_t dn[CPL], frn[CPL], rin[CPL], can[CPL];
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {
+                const int bc = min(tid + q * NT, n - 1);
+                dn[q] = D[(int64_t)inext * n + bc];
+                frn[q] = F[(int64_t)a_nx * n + bc];
+                rin[q] = Ig[(int64_t)inext * ld + bc];
+                can[q] = Ig[(int64_t)bc * ld + a_nx];
+            }
+
+            // (2)-(3) heat-bath pick; the winner commits the permutation swap
+            const Pick pk = heat_bath_pick<NT, CPL>(dE, lmax, pv, ri, beta2f, c20, a, i, red_s, tid);
+            int32_t my_rv = 0, my_cv = 0;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q)
+                if (pk.owner && pk.b == tid + q * NT) {
+                    my_rv = rv[q];
+                    my_cv = cv[q];
+                }
+            if (pk.owner && pk.b != a) {
+                p_s[i] = pk.b;
+                p_s[pk.j] = a;
+            }
+            {
+                int32_t *dnb = drow_s + (buf ^ 1) * n4;
+                int32_t *fnb = frow_s + (buf ^ 1) * n4;
+                int32_t *rnb = rowI_s + (buf ^ 1) * n4;
+                int32_t *cnb = colA_s + (buf ^ 1) * n4;
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const int bb = tid + q * NT;
+                    if (bb < n) {
+                        dnb[bb] = dn[q];
+                        fnb[bb] = frn[q];
+                        rnb[bb] = rin[q];
+                        cnb[bb] = can[q];
+                    }
+                }
+            }
+            if (pk.owner) {
+                red_s[96] = pk.b;
+                red_s[97] = pk.d;
+                red_s[98] = pk.j;
+                red_s[99] = my_cv;  // I[j][a] before the move
+            }
+            if (pk.S == 0 && tid == 0) {  // every weight truncated to zero: stay
+                red_s[96] = a;
+                red_s[97] = 0;
+                red_s[98] = i;
+                red_s[99] = 0;
+            }
+            lds_sync<NW>();
+            const int b = red_s[96], j = red_s[98];
+            const int32_t dsel = red_s[97], cv_w = red_s[99];
+            if (b != a) {
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const int bb = tid + q * NT;
+                    if (bb == b) pv[q] = i;
+                    else if (bb == a) pv[q] = j;
+                }
+                E -= dsel;  // (4) commit (CPU_PIM.cpp:252-283)
+                ++acc;
+                if (cls_s[a] != cls_s[b]) {
+                    // the move's u (rows) and v (columns) become pending update K
+                    uint32_t cnt = 0;
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q) {
+                        const int c = tid + q * NT;
+                        const int cc = min(c, n - 1);
+                        const int32_t fb = F[(int64_t)b * n + cc];
+                        const int32_t dj = D[(int64_t)j * n + cc];
+                        const int32_t v = 2 * (fb - frow[cc]);   // 2 (F[b][c] - F[a][c])
+                        if (c < n) {
+                            U_s[(size_t)c * kpad + K] = (int16_t)(dj - drow[c]);  // D[j][c] - D[i][c]
+                            V_s[(size_t)c * kpad + K] = (int16_t)v;
+                        }
+                        cnt += (uint32_t)__popcll(__ballot(c < n && v != 0));
+                    }
+                    cells += (uint64_t)n * cnt;
+                    lds_sync<NW>();
+                    // diagonal entries after the move: the swapped facilities take the
+                    // entries at their new locations, the others add u[pinv(c)] v[c]
+#pragma unroll
+                    for (int q = 0; q < CPL; ++q) {
+                        const int c = tid + q * NT;
+                        if (c < n) {
+                            const int32_t vc = V_s[(size_t)c * kpad + K];
+                            if (c == b) dg_s[c] = my_rv + (int32_t)U_s[(size_t)i * kpad + K] * vc;
+                            else if (c == a) dg_s[c] = cv_w + (int32_t)U_s[(size_t)j * kpad + K] * vc;
+                            else dg_s[c] += (int32_t)U_s[(size_t)pv[q] * kpad + K] * vc;
+                        }
+                    }
+                    ++K;
+                    lds_sync<NW>();
+                    if (K == kmax) flush();
+                } else if (tid == 0) {
+                    // identical flow rows and columns: I's columns a and b are equal, so the
+                    // diagonal entries of the two facilities trade places
+                    const int32_t t = dg_s[a];
+                    dg_s[a] = dg_s[b];
+                    dg_s[b] = t;
+                }
+                if (cls_s[a] == cls_s[b]) lds_sync<NW>();
+                if (E < bestE) {  // CPU_PIM.cpp:271-272, plus the permutation
+                    bestE = E;
+                    for (int x = tid; x < n; x += NT) bestpg[x] = p_s[x];
+                }
+            }
+            buf ^= 1;
+            a_cur = a_nx;
+            a_reg = (b != a && j == inext) ? a : a_nx;
+        }
+    }
+    flush();
+
+    if (lane == 0) reinterpret_cast<uint64_t *>(red_s + 140)[wid] = cells;
+    lds_sync<NW>();
+    for (int x = tid; x < n; x += NT) pg[x] = p_s[x];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+        const int bb = tid + q * NT;
+        if (bb < n) pinvg[bb] = pv[q];
+    }
+    if (tid == 0) {
+        uint64_t tot = cells;
+        if constexpr (NW > 1) {
+            const uint64_t *wc = reinterpret_cast<const uint64_t *>(red_s + 140);
+#pragma unroll
+            for (int w = 1; w < NW; ++w) tot += wc[w];
+        }
+        P.E[g] = E;
+        P.bestE[g] = bestE;
+        P.accepted[g] = acc;
+        P.cells[g] += tot;
+    }
+}
+
+template <int NT, int CPL>
+hipError_t launch_lazy_t(const PoolDev &P, const int32_t *list, int nblocks, int lds_bytes,
+                         hipStream_t s) {
+    auto k = sweep_lazy_kernel<NT, CPL>;
+    static bool attr_done = false;
+    if (!attr_done) {
+        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  kMaxLdsBytes);
+        attr_done = true;
+    }
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds_bytes, s, P, list, lds_bytes);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_lazy(const PoolDev &P, const SweepShape &sh, const int32_t *list,
+                             int nblocks, int lds_bytes, hipStream_t s) {
+    if (!sh.sym) return hipErrorInvalidValue;
+    switch (sh.nt * 8 + sh.cpl) {
+    case 64 * 8 + 1: return launch_lazy_t<64, 1>(P, list, nblocks, lds_bytes, s);
+    case 64 * 8 + 2: return launch_lazy_t<64, 2>(P, list, nblocks, lds_bytes, s);
+    case 64 * 8 + 4: return launch_lazy_t<64, 4>(P, list, nblocks, lds_bytes, s);
+    case 128 * 8 + 1: return launch_lazy_t<128, 1>(P, list, nblocks, lds_bytes, s);
+    case 128 * 8 + 2: return launch_lazy_t<128, 2>(P, list, nblocks, lds_bytes, s);
+    case 256 * 8 + 1: return launch_lazy_t<256, 1>(P, list, nblocks, lds_bytes, s);
+    case 256 * 8 + 2: return launch_lazy_t<256, 2>(P, list, nblocks, lds_bytes, s);
+    case 256 * 8 + 4: return launch_lazy_t<256, 4>(P, list, nblocks, lds_bytes, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace qap

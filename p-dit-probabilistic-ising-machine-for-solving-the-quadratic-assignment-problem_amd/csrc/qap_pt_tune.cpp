@@ -2,7 +2,7 @@
 // on the device it describes; read here by the shape choice at pool creation).
 //
 // Format, one row per line, '#' starts a comment:
-//   n_max  dense|sparse  lds32|lds16|hbm  nt  cpl  [swap-attempts/s]
+//   n_max  dense|sparse  lds32|lds16|lazy|hbm  nt  cpl  [swap-attempts/s]
 // Rows of one flow class and placement are taken in increasing n_max; an instance uses
 // the first row of its class and placement with n <= n_max.  The optional rate lets the
 // placement choice compare placements that all fit.
@@ -54,6 +54,7 @@ void load_table() {
         if (place == "lds32") t.im = IM_LDS32;
         else if (place == "lds16") t.im = IM_LDS16;
         else if (place == "hbm") t.im = IM_HBM;
+        else if (place == "lazy") t.im = IM_LAZY;
         else continue;
         if (t.n_max < 1 || !sweep_shape_supported(t.nt, t.cpl)) continue;
         g_table.push_back(t);

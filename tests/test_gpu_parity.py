@@ -97,6 +97,15 @@ CASES = {
                        {"env": {"QAP_PT_SHAPE": "64,4"}}),
     "grey_n256_R4_hbm": (lambda q: q.instances.grey_c2(), 4, 2, {"env": {"QAP_PT_IMODE": "2"}}),
     "n300_256x2_hbm": (lambda q: q.instances.mixed_instance(18, 300, True), 2, 1, {}),
+    # force matrix in HBM with rank-2 updates batched in LDS (symmetric instances)
+    "lazy_n40_R4": (lambda q: q.instances.random_instance(29, 40), 4, 12,
+                    {"env": {"QAP_PT_IMODE": "3"}}),
+    "lazy_sparse_n90_64x2": (lambda q: q.instances.mixed_instance(30, 90, False), 4, 8,
+                             {"env": {"QAP_PT_IMODE": "3", "QAP_PT_SHAPE": "64,2"}}),
+    "lazy_grey_n256_R3": (lambda q: q.instances.grey_c2(), 3, 3, {"env": {"QAP_PT_IMODE": "3"}}),
+    "lazy_n200_T2_kick": (lambda q: q.instances.mixed_instance(31, 200, True), 3, 4,
+                          {"T": 2, "kick_after": 1, "env": {"QAP_PT_IMODE": "3"}}),
+    "lazy_c5_n512": (lambda q: q.instances.large_c5(512), 2, 1, {}),
 }
 
 

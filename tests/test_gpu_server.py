@@ -61,3 +61,36 @@ def test_budget_serialises_admission(qpt, oracle, gpu):
     for k in range(len(reqs)):
         assert done[k]["status"] == qpt.QAP_SRV_DONE
         assert done[k]["best_cost"] == alone[k]["best_cost"]
+
+
+def test_banks_mixed_classes_goal_flight_and_empty_run(qpt, gpu):
+    """Requests of several bank classes (n up to 300), more of them than a bank has slots,
+    one with a goal (halts early), one with more replicas than any bank slot (runs as a
+    pool of its own) and one with no sweeps: every result equals a standalone solve."""
+    reqs = []
+    for k, n in enumerate([20, 70, 130, 200, 300, 45, 64, 129, 90, 33]):
+        F, D = qpt.instances.mixed_instance(800 + k, n, dense=(k % 2 == 1))
+        betas = qpt.instances.geometric_ladder(8, 1.0, 100.0) / max(1.0, n / 16.0)
+        reqs.append(qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=4 + 2 * (k % 3), seed=1200 + k))
+    F, D = qpt.instances.mixed_instance(850, 40, dense=True)
+    free = qpt.solve(qpt.Instance(F=F, D=D, beta_ln=qpt.instances.geometric_ladder(8, 1.0, 20.0),
+                                  sweeps=40, seed=1300))
+    reqs.append(qpt.Instance(F=F, D=D, beta_ln=qpt.instances.geometric_ladder(8, 1.0, 20.0),
+                             sweeps=40, seed=1300, goal=int(free["best_cost"] * 1.05)))
+    reqs.append(qpt.Instance(F=F, D=D, beta_ln=qpt.instances.geometric_ladder(16, 1.0, 20.0),
+                             sweeps=6, T=5, seed=1301))                       # 80 replicas
+    reqs.append(qpt.Instance(F=F, D=D, beta_ln=qpt.instances.geometric_ladder(8, 1.0, 20.0),
+                             sweeps=0, seed=1302))                            # no sweeps
+    alone = [qpt.solve(r) for r in reqs]
+    with qpt.Server(max_pools=8, max_batch=3) as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        got = [srv.wait(t, timeout_s=120.0) for t in tickets]
+    for k in range(len(reqs)):
+        print(f"request {k}: n={reqs[k].n} status={got[k] and got[k]['status']}", flush=True)
+        assert got[k] is not None and got[k]["status"] == qpt.QAP_SRV_DONE
+        assert got[k]["best_cost"] == alone[k]["best_cost"]
+        np.testing.assert_array_equal(got[k]["best_perm"], alone[k]["best_perm"])
+        assert got[k]["accepted"] == alone[k]["accepted"]
+        assert got[k]["sweeps_done"] == alone[k]["sweeps_done"]
+        assert got[k]["exchanges"] == alone[k]["exchanges"]
+    assert alone[-4]["sweeps_done"] < 40          # the goal stopped it early

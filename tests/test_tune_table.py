@@ -41,8 +41,8 @@ def test_size_rule_without_table(tmp_path):
     assert s["16"].startswith("64x1,lds32")
     assert s["65"].startswith("128x1,lds32")
     assert s["200"].startswith("256x1,")
-    assert s["300"].startswith("256x2,hbm")
-    assert s["700"].startswith("256x4,hbm")
+    assert s["300"].startswith("256x2,lazy")   # symmetric, too big for LDS
+    assert s["700"].startswith("256x4,lazy")
 
 
 def test_table_rows_apply_by_size_flow_and_placement(tmp_path):
@@ -51,15 +51,15 @@ def test_table_rows_apply_by_size_flow_and_placement(tmp_path):
         64 dense lds32 64 1   # trailing comment
         128 dense lds32 64 2
         256 dense hbm 128 2
-        1024 dense hbm 256 4
+        1024 dense lazy 256 4
         128 sparse lds32 128 1
         """)
     s = _shapes(tmp_path, table)
     assert s["40"].startswith("64x1,lds32")
     assert s["100"].startswith("64x2,lds32")      # second dense lds32 row
     assert s["129"].startswith("256x1,")          # lds32 row ends at 128; size rule
-    assert s["300"].startswith("256x4,hbm")       # 1024-row of the hbm placement
-    assert s["700"].startswith("256x4,hbm")
+    assert s["300"].startswith("256x4,lazy")      # 1024-row of the lazy placement
+    assert s["700"].startswith("256x4,lazy")
 
 
 def test_bad_rows_are_ignored_and_shapes_must_cover_n(tmp_path):
@@ -83,10 +83,12 @@ def test_environment_shape_overrides_table(tmp_path):
 
 
 def test_rates_choose_among_placements_that_fit(tmp_path):
-    # n=64 fits the int32 LDS placement; a table that rates the in-HBM placement higher
-    # for n <= 64 moves it there, while n=100 (no hbm row rated) keeps the default order.
+    # n=64 fits the int32 LDS placement (and the lazy one: the probe is symmetric); a
+    # table that rates the in-HBM placement highest for n <= 64 moves it there, while
+    # n=100 (no hbm or lazy row rated) keeps the default order.
     table = textwrap.dedent("""\
         64 dense lds32 64 1 1.0e8
+        64 dense lazy 64 1 0.5e8
         64 dense hbm 64 1 2.0e8
         128 dense lds32 128 1 1.0e8
         """)
