@@ -189,9 +189,12 @@ int qap_pt_auto_beta(const qap_problem *prob, const qap_pt_params *base, double 
 int qap_pt_auto_select(const double *avg_energy, int32_t steps, double lower_prop,
                        double higher_prop, int32_t R, double *beta0_ln, double *ratio);
 
-/* Sweep-kernel shape chosen for a symmetric instance of size n with a large force
- * bound ("threads x candidates per thread, lds32|lds16|hbm"), for reports. */
-int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen);
+/* Sweep-kernel shape chosen for a symmetric instance of size n with a large force bound
+ * in a pool of `replicas` replicas in all ("threads x candidates per thread,
+ * lds32|lds16|lazy|hbm"), for reports.  Pools with more replicas than the device has CUs
+ * take the placement with the best tuned full-chip rate; smaller ones the lowest-latency
+ * placement that fits. */
+int qap_pt_kernel_shape(int32_t n, int64_t replicas, char *buf, int32_t buflen);
 /* Shape actually used for instance inst of a pool ("NTxCPL,placement,sym|general,
  * dense-flow|sparse-flow"). */
 int qap_pt_pool_shape(qap_pt_pool *pool, int32_t inst, char *buf, int32_t buflen);

@@ -145,7 +145,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.qap_pt_pool_best_costs.argtypes = [C.c_void_p, _I64P]
     lib.qap_pt_pool_verify.argtypes = [C.c_void_p, _I64P, _I64P]
     lib.qap_pt_pool_destroy.argtypes = [C.c_void_p]
-    lib.qap_pt_kernel_shape.argtypes = [C.c_int32, C.c_char_p, C.c_int32]
+    lib.qap_pt_kernel_shape.argtypes = [C.c_int32, C.c_int64, C.c_char_p, C.c_int32]
     lib.qap_pt_pool_shape.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]
     lib.qap_pt_auto_beta.argtypes = [C.POINTER(QapProblem), C.POINTER(QapPtParams), C.c_double,
                                      C.c_double, _F64P, _F64P, _F64P]
@@ -173,9 +173,10 @@ def device_count() -> int:
     return int(load_library().qap_pt_device_count())
 
 
-def kernel_shape(n: int) -> str:
+def kernel_shape(n: int, replicas: int = 1) -> str:
+    """Shape and placement for a symmetric size-n instance in a pool of `replicas`."""
     buf = C.create_string_buffer(64)
-    _check(load_library().qap_pt_kernel_shape(n, buf, 64))
+    _check(load_library().qap_pt_kernel_shape(n, replicas, buf, 64))
     return buf.value.decode()
 
 

@@ -107,6 +107,21 @@ int round4i(int x) { return (x + 3) & ~3; }
 namespace qap {
 int hbm_stride(int n) { return round4i(n + 1); }
 
+int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            cus = v;
+        else
+            cus = 256;  // MI355X, for shape reports without a device
+    }
+    return cus;
+}
+
+bool throughput_regime(int64_t replicas) { return replicas > device_cus(); }
+
 int lds_stride(int n, int im) {
     if (im == IM_LDS32) {  // 16-byte rows for 128-bit updates; odd number of 4-word quads
         int q = (n + 1 + 3) / 4;
@@ -121,13 +136,14 @@ int lds_stride(int n, int im) {
 namespace {
 
 // Threads x candidates per thread and force-matrix placement for size n.
-// Placement: QAP_PT_IMODE=0|1|2 when that one fits (LDS budget, uint16 range); else the
-// fitting placement with the highest tuned rate when the table rates all of them; else
-// the first of lds32 -> lds16 -> hbm that fits.  Shape: QAP_PT_SHAPE="nt,cpl" (tests and
+// Placement: QAP_PT_IMODE=0|1|2|3 when that one fits (LDS budget, uint16 range, lazy
+// conditions); else, for a throughput-bound pool, the fitting placement with the highest
+// tuned rate when the table rates all of them; else the first of lds32 -> lds16 -> lazy ->
+// hbm that fits.  Shape: QAP_PT_SHAPE="nt,cpl" (tests and
 // the autotuner), else the tuned table's row for (n, flow class, placement), else a size
 // rule.
 SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool sparse,
-                        bool small_uv, int *ld_s) {
+                        bool small_uv, bool throughput, int *ld_s) {
     SweepShape s{};
     s.sym = sym;
     const bool fit32 = sweep_lds_bytes(n, lds_stride(n, IM_LDS32), IM_LDS32) <= kMaxLdsBytes;
@@ -144,9 +160,11 @@ SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool 
     else if (fit16) s.im = IM_LDS16;
     else if (lazy) s.im = IM_LAZY;
     else s.im = IM_HBM;
-    if (want < 0) {
-        // With tuned rates for every placement that fits, take the fastest (a uint16
-        // matrix halves the LDS footprint, so more replicas share a CU).
+    if (want < 0 && throughput) {
+        // A pool with more replicas than CUs is throughput-bound: with tuned rates for
+        // every placement that fits, take the fastest (a smaller LDS footprint lets more
+        // replicas share a CU).  A smaller pool is latency-bound and keeps the order above,
+        // which puts the force matrix in LDS whenever it fits.
         const int cand[4] = {fit32 ? IM_LDS32 : -1, fit16 ? IM_LDS16 : -1, lazy ? IM_LAZY : -1,
                              IM_HBM};
         double best_rate = 0.0;
@@ -399,7 +417,8 @@ int build_graph(qap_pt_pool *p, int depth) {
 }  // namespace
 namespace qap {
 
-void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, HostInstance &h) {
+void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throughput,
+                      HostInstance &h) {
     const int n = pb.n;
     const int64_t nn = (int64_t)n * n;
     InstDesc &d = h.d;
@@ -411,7 +430,7 @@ void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, HostInstan
     const bool sparse = offdiag_density(n, pb.F) < kSparseFlow;
     // lazy updates hold u (|u| <= 2 max|D|) and v (|v| <= 4 max|F|) in 16 bits
     const bool small_uv = 2 * bnd.maxD <= 32767 && 4 * bnd.maxF <= 32767;
-    h.shape = choose_shape(n, sym, nonneg, bnd.force, sparse, small_uv, &ld_s);
+    h.shape = choose_shape(n, sym, nonneg, bnd.force, sparse, small_uv, throughput, &ld_s);
     d = InstDesc{};
     d.n = n;
     d.ld = hbm_stride(n);
@@ -551,9 +570,12 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     std::vector<double> c20;
     std::map<std::tuple<int, int, int, int>, int> bucket_of;
     auto append = [](auto &dst, const auto &src) { dst.insert(dst.end(), src.begin(), src.end()); };
+    int64_t all_reps = 0;
+    for (int k = 0; k < count; ++k) all_reps += (int64_t)params[k].R * params[k].T;
+    const bool throughput = throughput_regime(all_reps);
     for (int k = 0; k < count; ++k) {
         HostInstance h;
-        prepare_instance(problems[k], params[k], h);
+        prepare_instance(problems[k], params[k], throughput, h);
         InstDesc &d = p->desc[k];
         d = h.d;
         p->shapes[k] = h.shape;
@@ -1085,10 +1107,12 @@ int qap_pt_pool_shape(qap_pt_pool *p, int32_t inst, char *buf, int32_t buflen) {
     return QAP_PT_OK;
 }
 
-int qap_pt_kernel_shape(int32_t n, char *buf, int32_t buflen) {
-    if (n < 1 || n > QAP_PT_MAX_N || !buf || buflen < 1) return fail(QAP_PT_E_INVAL, "bad argument");
+int qap_pt_kernel_shape(int32_t n, int64_t replicas, char *buf, int32_t buflen) {
+    if (n < 1 || n > QAP_PT_MAX_N || replicas < 1 || !buf || buflen < 1)
+        return fail(QAP_PT_E_INVAL, "bad argument");
     int ld_s = 0;
-    const SweepShape s = choose_shape(n, true, true, (int64_t)1 << 40, false, true, &ld_s);
+    const SweepShape s = choose_shape(n, true, true, (int64_t)1 << 40, false, true,
+                                      throughput_regime(replicas), &ld_s);
     std::snprintf(buf, buflen, "%dx%d,%s", s.nt, s.cpl,
                   placement_name(s.im));
     return QAP_PT_OK;

@@ -93,9 +93,14 @@ hipError_t dalloc(T **p, size_t count) {
 }
 
 int validate(const qap_problem &pb, const qap_pt_params &pr, int k);
-void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, HostInstance &h);
+// throughput: the instance runs in a pool with more replicas than the device has CUs (the
+// placement is then chosen by tuned full-chip rate rather than by per-replica latency)
+void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throughput,
+                      HostInstance &h);
 // Row stride of a force matrix in HBM for size n.
 int hbm_stride(int n);
+int device_cus();
+bool throughput_regime(int64_t replicas);
 // Orders a bucket's workgroups: larger n first, groups of 8 instances interleaved
 // replica by replica (consecutive workgroup ids go round-robin over the 8 XCDs, so each
 // instance stays on one XCD and its F and D stay in that XCD's L2).

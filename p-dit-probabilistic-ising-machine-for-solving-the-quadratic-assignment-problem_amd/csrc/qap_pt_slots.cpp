@@ -161,7 +161,8 @@ int slots_load(qap_pt_pool *p, int slot, const qap_problem &pb, const qap_pt_par
     QAP_HIP_TRY(hipSetDevice(p->device));
     HostInstance &h = p->staged[slot];
     h = HostInstance{};
-    prepare_instance(pb, pr, h);
+    // a bank is sized to run full: its slots' placement follows the full-chip rates
+    prepare_instance(pb, pr, throughput_regime((int64_t)p->count * p->rt_cap), h);
     const SlotLayout L = layout(p->n_cap, p->r_cap, p->t_cap, p->rt_cap);
     InstDesc &d = h.d;
     const int n = d.n, RT = d.R * d.T;

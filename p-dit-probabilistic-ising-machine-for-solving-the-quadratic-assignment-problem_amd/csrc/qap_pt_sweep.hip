@@ -269,49 +269,45 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         fsp_s[x] = P.fspan[d.vec_off + x];
         if constexpr (!SYM) ddiag_s[x] = Ddg[x];
     }
-    // HBM rows are 16-byte aligned (ld % 4 == 0): a (row, 4-column chunk) tiling of the
-    // workgroup with 8 chunk loads in flight per thread stages I into LDS.
-    const int L4 = ld >> 2;
-    int cw_st = 1;
-    while (cw_st < L4 && cw_st < NT) cw_st <<= 1;
-    const int sh_st = __builtin_ctz(cw_st);
-    const int rw_st = NT >> sh_st;
-    const int cx_st = tid & (cw_st - 1), ry_st = tid >> sh_st;
-    constexpr int kStageU = 8;
-    if constexpr (IM == IM_LDS32) {  // 16-byte rows on both sides: straight quad copies
-        // (a native vector type: HIP's int4 struct copies are not promoted to registers)
-        const v4i *Ig4 = reinterpret_cast<const v4i *>(Ig);
-        v4i *Is4 = reinterpret_cast<v4i *>(I_s);
-        const int ld4 = ld_s >> 2;
-        for (int cx = cx_st; cx < L4; cx += cw_st) {
-            for (int k0 = ry_st; k0 < n; k0 += rw_st * kStageU) {
-                v4i v[kStageU];
-#pragma unroll
-                for (int u = 0; u < kStageU; ++u) v[u] = Ig4[(int64_t)min(k0 + u * rw_st, n - 1) * L4 + cx];
-#pragma unroll
-                for (int u = 0; u < kStageU; ++u) {
-                    const int k = k0 + u * rw_st;
-                    if (k < n) Is4[k * ld4 + cx] = v[u];
-                }
-            }
+    // HBM rows are 16-byte aligned (ld % 4 == 0): the matrix is n * L4 quads, walked flat
+    // by the whole workgroup (quad g = tid + m NT at row g / L4, column quad g % L4) with
+    // kStageU loads in flight per thread.
+    const int L4 = ld >> 2, Q = n * L4;
+    const int dq = NT / L4, dr = NT - dq * L4;  // advance of (row, quad) per NT quads
+    const int row_t = tid / L4, cx_t = tid - row_t * L4;
+    auto advance = [&](int &row, int &cx) {
+        cx += dr;
+        row += dq;
+        if (cx >= L4) {
+            cx -= L4;
+            ++row;
         }
-    } else if constexpr (IM == IM_LDS16) {  // 4 int32 -> 4 uint16 (-I), 8-byte LDS stores
-        const int4 *Ig4 = reinterpret_cast<const int4 *>(Ig);
-        uint2 *Is2 = reinterpret_cast<uint2 *>(I_s);
-        const int ld2 = ld_s >> 2;
-        for (int cx = cx_st; cx < L4; cx += cw_st) {
-            for (int k0 = ry_st; k0 < n; k0 += rw_st * kStageU) {
-                int4 v[kStageU];
+    };
+    constexpr int kStageU = 16;
+    if constexpr (IM == IM_LDS32 || IM == IM_LDS16) {
+        // (native vector types: HIP's int4 struct copies are not promoted to registers)
+        const v4i *Ig4 = reinterpret_cast<const v4i *>(Ig);
+        const int ldq = ld_s >> 2;  // LDS row stride in 16-byte (int32) / 8-byte (uint16) units
+        int row = row_t, cx = cx_t;
+        for (int g0 = tid; g0 < Q; g0 += kStageU * NT) {
+            v4i v[kStageU];
+            int ro[kStageU], co[kStageU];
 #pragma unroll
-                for (int u = 0; u < kStageU; ++u) v[u] = Ig4[(int64_t)min(k0 + u * rw_st, n - 1) * L4 + cx];
+            for (int u = 0; u < kStageU; ++u) {
+                ro[u] = row;
+                co[u] = cx;
+                v[u] = Ig4[(int64_t)min(row, n - 1) * L4 + cx];
+                advance(row, cx);
+            }
 #pragma unroll
-                for (int u = 0; u < kStageU; ++u) {
-                    const int k = k0 + u * rw_st;
-                    if (k < n) {
-                        const uint32_t w0 = ((uint32_t)(-v[u].x) & 0xffffu) | ((uint32_t)(-v[u].y) << 16);
-                        const uint32_t w1 = ((uint32_t)(-v[u].z) & 0xffffu) | ((uint32_t)(-v[u].w) << 16);
-                        Is2[k * ld2 + cx] = uint2{w0, w1};
-                    }
+            for (int u = 0; u < kStageU; ++u) {
+                if (ro[u] >= n) continue;
+                if constexpr (IM == IM_LDS32) {
+                    reinterpret_cast<v4i *>(I_s)[ro[u] * ldq + co[u]] = v[u];
+                } else {  // 4 int32 -> 4 uint16 holding -I, one 8-byte store
+                    const uint32_t w0 = ((uint32_t)(-v[u].x) & 0xffffu) | ((uint32_t)(-v[u].y) << 16);
+                    const uint32_t w1 = ((uint32_t)(-v[u].z) & 0xffffu) | ((uint32_t)(-v[u].w) << 16);
+                    reinterpret_cast<uint2 *>(I_s)[ro[u] * ldq + co[u]] = uint2{w0, w1};
                 }
             }
         }
@@ -724,27 +720,25 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         const int bb = tid + q * NT;
         if (bb < n) pinvg[bb] = pv[q];
     }
-    if constexpr (IM == IM_LDS32) {
+    if constexpr (IM == IM_LDS32 || IM == IM_LDS16) {  // write the matrix back, flat as staged
         v4i *Ig4 = reinterpret_cast<v4i *>(Ig);
-        const v4i *Is4 = reinterpret_cast<const v4i *>(I_s);
-        const int ld4 = ld_s >> 2;
-        for (int cx = cx_st; cx < L4; cx += cw_st)
-            for (int k = ry_st; k < n; k += rw_st) Ig4[(int64_t)k * L4 + cx] = Is4[k * ld4 + cx];
-    } else if constexpr (IM == IM_LDS16) {
-        int4 *Ig4 = reinterpret_cast<int4 *>(Ig);
-        const uint2 *Is2 = reinterpret_cast<const uint2 *>(I_s);
-        const int ld2 = ld_s >> 2;
-        for (int cx = cx_st; cx < L4; cx += cw_st)
-            for (int k = ry_st; k < n; k += rw_st) {
-                const uint2 w = Is2[k * ld2 + cx];
+        const int ldq = ld_s >> 2;
+        int row = row_t, cx = cx_t;
+        for (int gq = tid; gq < Q; gq += NT) {
+            v4i o;
+            if constexpr (IM == IM_LDS32) {
+                o = reinterpret_cast<const v4i *>(I_s)[row * ldq + cx];
+            } else {
+                const uint2 w = reinterpret_cast<const uint2 *>(I_s)[row * ldq + cx];
                 const int c = 4 * cx;
-                int4 o;
                 o.x = (c + 0 < n) ? -(int32_t)(w.x & 0xffffu) : 0;
                 o.y = (c + 1 < n) ? -(int32_t)(w.x >> 16) : 0;
                 o.z = (c + 2 < n) ? -(int32_t)(w.y & 0xffffu) : 0;
                 o.w = (c + 3 < n) ? -(int32_t)(w.y >> 16) : 0;
-                Ig4[(int64_t)k * L4 + cx] = o;
             }
+            Ig4[(int64_t)row * L4 + cx] = o;
+            advance(row, cx);
+        }
     }
     if (tid == 0) {
         P.E[g] = E;
@@ -775,17 +769,12 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P) {
     __shared__ int32_t pos_s[kExMaxRT];
     __shared__ int64_t gbest_s[64];
     __shared__ uint32_t nex_s[64], nkick_s[64];
-    __shared__ int go_s;
     const int inst = blockIdx.x;
     const int tid = threadIdx.x;
-    InstState &st = P.state[inst];
+    // Every load of the step is issued first, in one round (all of it is valid memory
+    // whether or not the instance still runs); the decision to run comes after.
     const InstDesc d = P.inst[inst];
-    if (tid == 0) {
-        go_s = !(st.halted || st.megas_done >= st.mega_limit);
-        if (!go_s) st.active = 0;
-    }
-    __syncthreads();
-    if (!go_s) return;
+    const InstState st = P.state[inst];
     const int R = d.R, T = d.T, RT = R * T;
     const bool staged = RT <= kExMaxRT;
     int64_t *E = staged ? E_s : P.E + d.rep_base;
@@ -799,15 +788,25 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P) {
             ord_s[x] = P.order[d.rep_base + x];
             pos_s[x] = P.pos[d.rep_base + x];
         }
-        __syncthreads();
+    }
+    int32_t tss_pf = 0;
+    int64_t lbest_pf = INT64_MAX;
+    if (tid < T) {
+        tss_pf = P.tss[d.ladder_base + tid];
+        lbest_pf = P.ladder_best[d.ladder_base + tid];
+    }
+    __syncthreads();
+    if (st.halted || st.megas_done >= st.mega_limit) {
+        if (tid == 0) P.state[inst].active = 0;
+        return;
     }
     int64_t gbest = INT64_MAX;
     uint32_t nex = 0, nkick = 0;
     for (int t = tid; t < T; t += 64) {
         const int L = d.ladder_base + t;
         const int base = t * R;
-        int32_t tss = P.tss[L] + 1;
-        int64_t lbest = P.ladder_best[L];
+        int32_t tss = (t == tid ? tss_pf : P.tss[L]) + 1;
+        int64_t lbest = t == tid ? lbest_pf : P.ladder_best[L];
         for (int l = 0; l < R; ++l) {
             const int64_t b = bE[base + l];
             if (b < lbest) {
@@ -856,16 +855,17 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P) {
             ne += nex_s[x];
             nk += nkick_s[x];
         }
-        st.global_best = gb;
-        st.exchanges += ne;
-        st.kicks += nk;
+        InstState &so = P.state[inst];
+        so.global_best = gb;
+        so.exchanges = st.exchanges + ne;
+        so.kicks = st.kicks + nk;
         if (d.has_goal && gb <= d.goal) {
-            st.halted = 1;
-            st.active = 0;
+            so.halted = 1;
+            so.active = 0;
         } else {
-            st.sweep_base = (uint32_t)(st.megas_done * d.spe);
-            st.megas_done += 1;
-            st.active = 1;
+            so.sweep_base = (uint32_t)(st.megas_done * d.spe);
+            so.megas_done = st.megas_done + 1;
+            so.active = 1;
         }
     }
 }

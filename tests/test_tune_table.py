@@ -16,7 +16,8 @@ PROBE = textwrap.dedent("""
     from conftest import load_qpt
     q = load_qpt()
     for n in (16, 40, 64, 65, 100, 129, 200, 300, 700):
-        print(n, q.kernel_shape(n))
+        print(n, "lat", q.kernel_shape(n))
+        print(n, "thr", q.kernel_shape(n, 4096))
 """)
 
 
@@ -33,7 +34,11 @@ def _shapes(tmp_path, table_text, extra_env=None):
     env.update(extra_env or {})
     out = subprocess.run([sys.executable, "-c", PROBE.format(tests=os.path.join(ROOT, "tests"))],
                          capture_output=True, text=True, env=env, timeout=120, check=True).stdout
-    return dict(line.split(" ", 1) for line in out.strip().splitlines())
+    shapes = {}
+    for line in out.strip().splitlines():
+        n, regime, shape = line.split(" ", 2)
+        shapes[n if regime == "lat" else n + "@full"] = shape
+    return shapes
 
 
 def test_size_rule_without_table(tmp_path):
@@ -83,9 +88,10 @@ def test_environment_shape_overrides_table(tmp_path):
 
 
 def test_rates_choose_among_placements_that_fit(tmp_path):
-    # n=64 fits the int32 LDS placement (and the lazy one: the probe is symmetric); a
-    # table that rates the in-HBM placement highest for n <= 64 moves it there, while
-    # n=100 (no hbm or lazy row rated) keeps the default order.
+    # n=64 fits the int32 LDS placement (and the lazy one: the probe is symmetric).  In a
+    # pool with more replicas than CUs, a table that rates the in-HBM placement highest
+    # for n <= 64 moves it there, while n=100 (no hbm or lazy row rated) keeps the default
+    # order; a small (latency-bound) pool keeps the LDS placement either way.
     table = textwrap.dedent("""\
         64 dense lds32 64 1 1.0e8
         64 dense lazy 64 1 0.5e8
@@ -93,8 +99,9 @@ def test_rates_choose_among_placements_that_fit(tmp_path):
         128 dense lds32 128 1 1.0e8
         """)
     s = _shapes(tmp_path, table)
-    assert s["40"].startswith("64x1,hbm")
-    assert s["100"].startswith("128x1,lds32")
+    assert s["40@full"].startswith("64x1,hbm")
+    assert s["100@full"].startswith("128x1,lds32")
+    assert s["40"].startswith("64x1,lds32")
     # an explicit placement request still wins
     s = _shapes(tmp_path, table, {"QAP_PT_IMODE": "0"})
-    assert s["40"].startswith("64x1,lds32")
+    assert s["40@full"].startswith("64x1,lds32")
