@@ -761,6 +761,23 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
 // (:346-348) uses the best over all ladders of the instance.
 // ---------------------------------------------------------------------------
 constexpr int kExMaxRT = 2048;
+constexpr int kExWaveLadders = 8;  // at most this many ladders of <= 64 rungs: wave path
+
+__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), lane);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// minimum over the wave, every lane gets it
+__device__ __forceinline__ int64_t wave_min64(int64_t v) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int64_t o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
 
 __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P) {
     __shared__ int64_t E_s[kExMaxRT];
@@ -802,40 +819,98 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P) {
     }
     int64_t gbest = INT64_MAX;
     uint32_t nex = 0, nkick = 0;
-    for (int t = tid; t < T; t += 64) {
-        const int L = d.ladder_base + t;
-        const int base = t * R;
-        int32_t tss = (t == tid ? tss_pf : P.tss[L]) + 1;
-        int64_t lbest = t == tid ? lbest_pf : P.ladder_best[L];
-        for (int l = 0; l < R; ++l) {
-            const int64_t b = bE[base + l];
-            if (b < lbest) {
-                lbest = b;
+    if (R <= kWave && T <= kExWaveLadders) {
+        // Few short ladders: the wave walks each ladder together, lane l holding ladder
+        // position l, so the sequential bubble pass runs on lane reads of registers
+        // instead of dependent LDS round trips (the same pass, step for step).
+        for (int t = 0; t < T; ++t) {
+            const int L = d.ladder_base + t;
+            const int base = t * R;
+            const bool live = tid < R;
+            const int r_l = live ? ord[base + tid] : 0;
+            const int64_t e_l = live ? E[base + r_l] : INT64_MAX;
+            const int64_t mb = wave_min64(live ? bE[base + tid] : INT64_MAX);
+            int32_t tss = __builtin_amdgcn_readlane(tss_pf, t) + 1;  // lane t prefetched ladder t
+            int64_t lbest = readlane64(lbest_pf, t);
+            if (mb < lbest) {  // best fold (CPU_PIM.cpp:297-321)
+                lbest = mb;
                 tss = 0;
             }
-        }
-        for (int k = 0; k < R - 1; ++k) {
-            const int r1 = ord[base + k], r2 = ord[base + k + 1];
-            if (E[base + r1] < E[base + r2]) {
-                ord[base + k] = r2;
-                ord[base + k + 1] = r1;
-                pos[base + r1] = k + 1;
-                pos[base + r2] = k;
-                ++nex;
-                if (r2 == R - 1) tss = 0;
+            // bubble pass (CPU_PIM.cpp:323-344): the element carried from position k
+            // meets the one at k + 1; the lower cost moves on, the other stays at k
+            int64_t cE = readlane64(e_l, 0);
+            int cR = __builtin_amdgcn_readlane(r_l, 0);
+            int newR = r_l;
+            for (int k = 0; k < R - 1; ++k) {
+                const int64_t nE = readlane64(e_l, k + 1);
+                const int nR = __builtin_amdgcn_readlane(r_l, k + 1);
+                if (cE < nE) {
+                    newR = tid == k ? nR : newR;
+                    ++nex;
+                    if (nR == R - 1) tss = 0;
+                } else {
+                    newR = tid == k ? cR : newR;
+                    cE = nE;
+                    cR = nR;
+                }
             }
+            newR = tid == R - 1 ? cR : newR;
+            if (tss > d.kick_after) {  // kick: the hottest and coldest positions trade
+                const int r0 = __builtin_amdgcn_readlane(newR, 0);
+                const int rL = __builtin_amdgcn_readlane(newR, R - 1);
+                newR = tid == 0 ? rL : (tid == R - 1 ? r0 : newR);
+                ++nkick;
+            }
+            if (live) {
+                ord[base + tid] = newR;
+                pos[base + newR] = tid;
+            }
+            if (tid == 0) {
+                P.tss[L] = tss;
+                P.ladder_best[L] = lbest;
+            }
+            gbest = lbest < gbest ? lbest : gbest;
         }
-        if (tss > d.kick_after) {
-            const int r1 = ord[base + R - 1], r2 = ord[base];
-            ord[base + R - 1] = r2;
-            ord[base] = r1;
-            pos[base + r1] = 0;
-            pos[base + r2] = R - 1;
-            ++nkick;
+        if (tid != 0) {  // the counts and best are wave-uniform: lane 0 reports them
+            gbest = INT64_MAX;
+            nex = nkick = 0;
         }
-        P.tss[L] = tss;
-        P.ladder_best[L] = lbest;
-        gbest = lbest < gbest ? lbest : gbest;
+    } else {
+        for (int t = tid; t < T; t += 64) {
+            const int L = d.ladder_base + t;
+            const int base = t * R;
+            int32_t tss = (t == tid ? tss_pf : P.tss[L]) + 1;
+            int64_t lbest = t == tid ? lbest_pf : P.ladder_best[L];
+            for (int l = 0; l < R; ++l) {
+                const int64_t b = bE[base + l];
+                if (b < lbest) {
+                    lbest = b;
+                    tss = 0;
+                }
+            }
+            for (int k = 0; k < R - 1; ++k) {
+                const int r1 = ord[base + k], r2 = ord[base + k + 1];
+                if (E[base + r1] < E[base + r2]) {
+                    ord[base + k] = r2;
+                    ord[base + k + 1] = r1;
+                    pos[base + r1] = k + 1;
+                    pos[base + r2] = k;
+                    ++nex;
+                    if (r2 == R - 1) tss = 0;
+                }
+            }
+            if (tss > d.kick_after) {
+                const int r1 = ord[base + R - 1], r2 = ord[base];
+                ord[base + R - 1] = r2;
+                ord[base] = r1;
+                pos[base + r1] = 0;
+                pos[base + r2] = R - 1;
+                ++nkick;
+            }
+            P.tss[L] = tss;
+            P.ladder_best[L] = lbest;
+            gbest = lbest < gbest ? lbest : gbest;
+        }
     }
     gbest_s[tid] = gbest;
     nex_s[tid] = nex;

@@ -89,6 +89,10 @@ CASES = {
     "per_replica_init_n33_T3": (lambda q: q.instances.random_instance(15, 33), 5, 10,
                                 {"T": 3, "per_replica_init": True}),
     "kick_often_n20": (lambda q: q.instances.random_instance(16, 20), 4, 60, {"kick_after": 2}),
+    # exchange step: many ladders (one thread per ladder) and a ladder longer than a wave
+    "ladders_T12_kick": (lambda q: q.instances.random_instance(32, 24), 4, 30,
+                         {"T": 12, "kick_after": 3}),
+    "long_ladder_R70": (lambda q: q.instances.random_instance(33, 16), 70, 8, {"kick_after": 2}),
     "multiwave_n64_256x1": (lambda q: q.instances.flat_c1(), 4, 6,
                             {"env": {"QAP_PT_SHAPE": "256,1"}}),
     "multiwave_n64_128x1": (lambda q: q.instances.flat_c1(), 4, 6,
