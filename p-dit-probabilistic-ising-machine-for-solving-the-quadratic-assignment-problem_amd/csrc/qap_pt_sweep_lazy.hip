@@ -33,13 +33,21 @@ constexpr int kLazyArrays = 13;
 
 __host__ __device__ inline int lazy_fixed_words(int n) { return kLazyArrays * round8(n) + kLazyRed; }
 
-// pending-update capacity a launch with `lds_bytes` of LDS gives an instance of size n
-// (a multiple of 8: U and V rows are read 8 entries at a time)
-__host__ __device__ inline int lazy_kmax(int n, int lds_bytes) {
+// Pending-update capacity (kmax, a multiple of 8: rows are read 8 entries at a time) and
+// U / V row stride (kstride int16) that `lds_bytes` of LDS give an instance of size n.
+// The stride is an odd number of 16-byte groups, so the 128-bit reads of consecutive
+// rows by consecutive lanes fall in distinct LDS banks.
+struct LazyLayout {
+    int kmax, kstride;
+};
+
+__host__ __device__ inline LazyLayout lazy_layout(int n, int lds_bytes) {
     const int avail = lds_bytes - 4 * lazy_fixed_words(n);
-    int k = avail / (4 * round8(n));  // U and V: 2 x 2 bytes per row entry
-    k = k < kLazyMaxK ? k : kLazyMaxK;
-    return k & ~7;
+    const int g = avail > 0 ? avail / (4 * round8(n) * 8) : 0;  // 8-entry groups that fit
+    int kg = g < kLazyMaxK / 8 ? g : kLazyMaxK / 8;
+    int sg = kg | 1;
+    if (sg > g) sg = --kg;  // kg was even and only kg groups fit: drop to kg - 1 (odd)
+    return LazyLayout{8 * (kg > 0 ? kg : 0), 8 * (sg > 0 ? sg : 0)};
 }
 
 using v4h = int __attribute__((ext_vector_type(4)));  // 8 packed int16
@@ -59,11 +67,12 @@ __device__ __forceinline__ int32_t dot8(v4h x, v4h y, int32_t acc) {
 }  // namespace
 
 int lazy_lds_bytes(int n) {
-    const int want = 4 * lazy_fixed_words(n) + 4 * round8(n) * kLazyMaxK;
+    // kLazyMaxK entries in an odd number of 8-entry groups
+    const int want = 4 * lazy_fixed_words(n) + 4 * round8(n) * 8 * ((kLazyMaxK / 8) | 1);
     return want < kMaxLdsBytes ? want : kMaxLdsBytes;
 }
 
-bool lazy_fits(int n) { return lazy_kmax(n, lazy_lds_bytes(n)) >= 8; }
+bool lazy_fits(int n) { return lazy_layout(n, lazy_lds_bytes(n)).kmax >= 8; }
 
 template <int NT, int CPL>
 __global__ __launch_bounds__(NT) void sweep_lazy_kernel(PoolDev P, const int32_t *__restrict__ rep_list,
@@ -78,9 +87,10 @@ __global__ __launch_bounds__(NT) void sweep_lazy_kernel(PoolDev P, const int32_t
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int n = d.n, ld = d.ld, n4 = round8(n);
     const int local = g - d.rep_base;
-    const int kmax = lazy_kmax(n, lds_bytes);
-    const int kpad = kmax;  // U / V row length in int16 (a multiple of 8)
-    if (kmax < 8) return;   // the host places an instance here only when 8 updates fit
+    const LazyLayout lay = lazy_layout(n, lds_bytes);
+    const int kmax = lay.kmax;
+    const int kpad = lay.kstride;  // U / V row stride in int16
+    if (kmax < 8) return;          // the host places an instance here only when 8 updates fit
 
     int32_t *p_s = smem;
     int32_t *pinv_s = p_s + n4;
