@@ -26,6 +26,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tune", "shapes_gfx950.txt")
 SHAPES = [(64, 1), (64, 2), (64, 4), (128, 1), (128, 2), (256, 1), (256, 2), (256, 4)]
+LAZY_SHAPES = [(512, 1), (512, 2)]  # eight-wave workgroups exist for the lazy kernel only
 PLACEMENTS = {0: "lds32", 1: "lds16", 2: "hbm", 3: "lazy"}
 DEFAULT_SIZES = [32, 48, 64, 96, 128, 160, 192, 256, 384, 512, 768, 1024]
 
@@ -41,8 +42,9 @@ def _load_pkg():
     return mod
 
 
-def candidate_shapes(n: int):
-    c = [(nt, cpl) for nt, cpl in SHAPES if n <= nt * cpl < 2 * n]
+def candidate_shapes(n: int, placement: int = 0):
+    shapes = SHAPES + (LAZY_SHAPES if placement == 3 else [])
+    c = [(nt, cpl) for nt, cpl in shapes if n <= nt * cpl < 2 * n]
     return c or [(64, 1)]
 
 
@@ -99,7 +101,7 @@ def main(argv=None):
             flow = "dense" if dense else "sparse"
             for placement in (0, 1, 2, 3):
                 rates = []
-                for shape in candidate_shapes(n):
+                for shape in candidate_shapes(n, placement):
                     rate, got = measure(pkg, n, dense, placement, shape, args.replicas,
                                         args.min_attempts)
                     print(f"[{time.time() - t0:7.1f}s] n={n} {flow} {PLACEMENTS[placement]} "

@@ -27,7 +27,8 @@ struct SweepShape {
     bool sym;    // symmetric, zero-diagonal F and D
 };
 
-bool sweep_shape_supported(int nt, int cpl);
+// whether a kernel instance exists for nt threads x cpl candidates under placement im
+bool sweep_shape_supported(int nt, int cpl, int im);
 int sweep_lds_bytes(int n, int ld_s, int im);
 // LDS of the lazy kernel for size n, and whether it holds at least 8 pending updates
 int lazy_lds_bytes(int n);

@@ -202,8 +202,12 @@ SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool 
         s.nt = 256;
         s.cpl = 4;
     }
+    if (s.im == IM_LAZY && n > 256) {  // eight waves keep more of the flush's loads in flight
+        s.nt = 512;
+        s.cpl = n <= 512 ? 1 : 2;
+    }
     auto take = [&](int nt, int cpl) {
-        if (sweep_shape_supported(nt, cpl) && nt * cpl >= n) {
+        if (sweep_shape_supported(nt, cpl, s.im) && nt * cpl >= n) {
             s.nt = nt;
             s.cpl = cpl;
         }
@@ -428,8 +432,9 @@ void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throu
     const Bounds bnd = compute_bounds(n, pb.F, pb.D);
     int ld_s = 0;
     const bool sparse = offdiag_density(n, pb.F) < kSparseFlow;
-    // lazy updates hold u (|u| <= 2 max|D|) and v (|v| <= 4 max|F|) in 16 bits
-    const bool small_uv = 2 * bnd.maxD <= 32767 && 4 * bnd.maxF <= 32767;
+    // lazy updates hold u (|u| <= 2 max|D|) and v (|v| <= 4 max|F|) in 16 bits, split into
+    // two signed bytes for the matrix cores (the high byte of 32640..32767 would not fit)
+    const bool small_uv = 2 * bnd.maxD <= 32639 && 4 * bnd.maxF <= 32639;
     h.shape = choose_shape(n, sym, nonneg, bnd.force, sparse, small_uv, throughput, &ld_s);
     d = InstDesc{};
     d.n = n;

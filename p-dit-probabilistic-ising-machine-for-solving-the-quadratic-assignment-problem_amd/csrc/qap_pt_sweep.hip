@@ -993,7 +993,8 @@ hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *l
     }
 }
 
-bool sweep_shape_supported(int nt, int cpl) {
+bool sweep_shape_supported(int nt, int cpl, int im) {
+    if (im == IM_LAZY && nt == 512 && (cpl == 1 || cpl == 2)) return true;
     switch (nt * 8 + cpl) {
     case 64 * 8 + 1: case 64 * 8 + 2: case 64 * 8 + 4: case 128 * 8 + 1: case 128 * 8 + 2:
     case 256 * 8 + 1: case 256 * 8 + 2: case 256 * 8 + 4: return true;

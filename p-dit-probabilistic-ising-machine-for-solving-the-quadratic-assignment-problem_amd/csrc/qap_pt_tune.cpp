@@ -56,7 +56,7 @@ void load_table() {
         else if (place == "hbm") t.im = IM_HBM;
         else if (place == "lazy") t.im = IM_LAZY;
         else continue;
-        if (t.n_max < 1 || !sweep_shape_supported(t.nt, t.cpl)) continue;
+        if (t.n_max < 1 || !sweep_shape_supported(t.nt, t.cpl, t.im)) continue;
         g_table.push_back(t);
     }
     std::stable_sort(g_table.begin(), g_table.end(),

@@ -110,6 +110,13 @@ CASES = {
     "lazy_n200_T2_kick": (lambda q: q.instances.mixed_instance(31, 200, True), 3, 4,
                           {"T": 2, "kick_after": 1, "env": {"QAP_PT_IMODE": "3"}}),
     "lazy_c5_n512": (lambda q: q.instances.large_c5(512), 2, 1, {}),
+    # eight waves per replica
+    "lazy_n200_512x1": (lambda q: q.instances.mixed_instance(33, 200, True), 3, 3,
+                        {"env": {"QAP_PT_IMODE": "3", "QAP_PT_SHAPE": "512,1"}}),
+    "lazy_c5_n512_512x1": (lambda q: q.instances.large_c5(512), 2, 1,
+                           {"env": {"QAP_PT_SHAPE": "512,1"}}),
+    "lazy_n700_512x2": (lambda q: q.instances.random_instance(34, 700), 2, 1,
+                        {"env": {"QAP_PT_IMODE": "3", "QAP_PT_SHAPE": "512,2"}}),
 }
 
 

@@ -46,8 +46,8 @@ def test_size_rule_without_table(tmp_path):
     assert s["16"].startswith("64x1,lds32")
     assert s["65"].startswith("128x1,lds32")
     assert s["200"].startswith("256x1,")
-    assert s["300"].startswith("256x2,lazy")   # symmetric, too big for LDS
-    assert s["700"].startswith("256x4,lazy")
+    assert s["300"].startswith("512x1,lazy")   # symmetric, too big for LDS: eight waves
+    assert s["700"].startswith("512x2,lazy")
 
 
 def test_table_rows_apply_by_size_flow_and_placement(tmp_path):
