@@ -46,6 +46,15 @@ extern "C" {
 /* qap_pt_params.flags */
 #define QAP_PT_INIT_PER_REPLICA 1u /* own initial shuffle per replica (GPU_PIM.cu:151-167)
                                       instead of one per ladder (CPU_PIM.cpp:143-161)     */
+#define QAP_PT_ONE_SHOT 2u         /* CUPIM's one-shot parallel-proposal dynamics
+                                      (GPU-CUPIM/perl-gpu/src/lib/qap/qap.cu:1037-1609,
+                                      ReplicaPool::perform_step_one_shot) instead of p-dit
+                                      sweeps: each "sweep" is one step in which every
+                                      location proposes a swap from the same snapshot,
+                                      conflicts resolve by the largest delta, and the
+                                      disjoint winners are applied together.  A different
+                                      Markov chain; attempts count n proposals per step.
+                                      Needs n <= 256, 2 max|D| <= 32639, 4 max|F| <= 32639. */
 
 /* One instance: n, then flow and distance, both n*n row-major int32, exactly as
  * readProblem stores them (CPU_PIM.cpp:79-107). */
@@ -176,7 +185,9 @@ int qap_srv_destroy(qap_srv *srv);
  * sweeps.  avg_energy[i] (may be NULL, else QAP_PT_AUTO_STEPS doubles) receives the mean
  * final cost at exponent i; beta0_ln and ratio the ladder for qap_pt_params as the
  * reference's -b / -m would give it (largest beta, multiplicative step).  base->beta_ln,
- * sweeps, has_goal, goal, timeout_s and init_perm are ignored. */
+ * sweeps, has_goal, goal, timeout_s and init_perm are ignored; base->flags chooses the
+ * trials' dynamics (QAP_PT_ONE_SHOT: one-shot steps, as GPU-CUPIM's automatic mode runs
+ * its own dynamics, qap_gpu.cu:502-720). */
 #define QAP_PT_AUTO_MIN_EXP (-30)
 #define QAP_PT_AUTO_STEPS 32
 #define QAP_PT_AUTO_TRIALS 5

@@ -19,6 +19,7 @@ REF_BIN = os.path.join(HERE, "_ref", "cpu_pim_ref")
 MODE_EXACT = 0
 MODE_REFFP = 1
 FLAG_INIT_PER_REPLICA = 1
+FLAG_ONE_SHOT = 2
 
 _I32P = C.POINTER(C.c_int32)
 _I64P = C.POINTER(C.c_int64)
@@ -74,6 +75,11 @@ def lib() -> C.CDLL:
                                            C.c_int32, C.c_float, C.c_double, C.c_uint32, _I32P,
                                            _U32P]
         L.oracle_attempt_exact.restype = C.c_int32
+        L.oracle_one_shot_step.argtypes = [C.c_int32, _I32P, _I32P, _I32P, _I32P, _I32P,
+                                           _I64P, C.c_float, C.c_double, C.c_uint64,
+                                           C.c_uint32, C.c_uint32, _I32P,
+                                           C.POINTER(C.c_uint64), _U32P, _I32P, _I32P]
+        L.oracle_one_shot_step.restype = C.c_int32
         L.oracle_beta2.argtypes = [C.c_double]
         L.oracle_beta2.restype = C.c_double
         L.oracle_init_perm.argtypes = [C.c_int32, C.c_uint64, C.c_uint32, _I32P]
@@ -163,6 +169,30 @@ def attempt_exact(F, D, perm, I, E, i, beta2f, c20, u):
     return int(sel), dE, w, p, I, int(e.value)
 
 
+def one_shot_step(F, D, perm, I, E, beta2f, c20, seed, step, stream):
+    """One one-shot parallel-proposal step on copies; returns
+    (applied, proposals, won, perm', I', E')."""
+    F = np.ascontiguousarray(F, np.int32)
+    D = np.ascontiguousarray(D, np.int32)
+    n = F.shape[0]
+    p = np.ascontiguousarray(perm, np.int32).copy()
+    pinv = np.argsort(p).astype(np.int32)
+    I = np.ascontiguousarray(I, np.int32).copy()
+    e = C.c_int64(int(E))
+    work = np.zeros(4 * n, np.int32)
+    owner = np.zeros(n, np.uint64)
+    w = np.zeros(n, np.uint32)
+    tgt = np.zeros(n, np.int32)
+    won = np.zeros(n, np.int32)
+    m = lib().oracle_one_shot_step(n, _p(F, C.c_int32), _p(D, C.c_int32), _p(p, C.c_int32),
+                                   _p(pinv, C.c_int32), _p(I, C.c_int32), C.byref(e),
+                                   C.c_float(beta2f), C.c_double(c20), C.c_uint64(seed),
+                                   C.c_uint32(step), C.c_uint32(stream), _p(work, C.c_int32),
+                                   _p(owner, C.c_uint64), _p(w, C.c_uint32),
+                                   _p(tgt, C.c_int32), _p(won, C.c_int32))
+    return int(m), tgt, won.astype(bool), p, I, int(e.value)
+
+
 def init_perm(n, seed, stream):
     p = np.zeros(n, np.int32)
     lib().oracle_init_perm(n, C.c_uint64(seed), C.c_uint32(stream), _p(p, C.c_int32))
@@ -180,7 +210,8 @@ def bounds(F, D):
 
 
 def solve(F, D, beta_ln, sweeps, R=None, T=1, spe=1, seed=0, kick_after=100, goal=None,
-          per_replica_init=False, init_perm=None, mode=MODE_EXACT, nthreads=1, traj=False):
+          per_replica_init=False, init_perm=None, mode=MODE_EXACT, nthreads=1, traj=False,
+          one_shot=False):
     """Full PT-SA solve; returns a dict with the result and the final replica state."""
     F = np.ascontiguousarray(F, np.int32)
     D = np.ascontiguousarray(D, np.int32)
@@ -197,7 +228,8 @@ def solve(F, D, beta_ln, sweeps, R=None, T=1, spe=1, seed=0, kick_after=100, goa
     par.kick_after = int(kick_after)
     par.has_goal = 0 if goal is None else 1
     par.goal = 0 if goal is None else int(goal)
-    par.flags = FLAG_INIT_PER_REPLICA if per_replica_init else 0
+    par.flags = (FLAG_INIT_PER_REPLICA if per_replica_init else 0) | \
+        (FLAG_ONE_SHOT if one_shot else 0)
     ip = None
     if init_perm is not None:
         ip = np.ascontiguousarray(init_perm, np.int32)

@@ -210,6 +210,81 @@ int32_t oracle_attempt_exact(int32_t n, const int32_t *F, const int32_t *D, int3
     return sel;
 }
 
+/* ------------------------------------------------------------------------- */
+/* One-shot parallel-proposal step (qap.cu:1037-1460), exact weights.         */
+/* ------------------------------------------------------------------------- */
+int32_t oracle_one_shot_step(int32_t n, const int32_t *F, const int32_t *D, int32_t *perm,
+                             int32_t *pinv, int32_t *I, int64_t *E, float beta2f, double c20,
+                             uint64_t seed, uint32_t step, uint32_t stream, int32_t *work,
+                             uint64_t *owner, uint32_t *w, int32_t *tgt, int32_t *acc) {
+    int32_t *prop = work, *del = work + n, *dE = work + 2 * n, *snap = work + 3 * n;
+    /* proposals, every one from the snapshot (qap.cu:1037-1140) */
+    for (int pi = 0; pi < n; pi++) {
+        int32_t maxd = 0;
+        for (int pj = 0; pj < n; pj++) {
+            dE[pj] = (int32_t)oracle_delta(n, F, D, perm, pinv, I, pi, perm[pj]);
+            if (dE[pj] > maxd) maxd = dE[pj];
+        }
+        int32_t cut = cutoff_of(maxd, c20);
+        uint32_t S = 0;
+        for (int pj = 0; pj < n; pj++) {
+            w[pj] = dE[pj] > cut ? oracle_weight(beta2f, dE[pj] - maxd) : 0u;
+            S += w[pj];
+        }
+        uint32_t x[4];
+        ph_draw(seed, (uint32_t)pi, step, stream, 3u, x);
+        uint32_t T = (uint32_t)(((uint64_t)x[0] * S) >> 32);
+        uint32_t P = 0;
+        int32_t sel = pi; /* every weight truncated to zero: stay */
+        for (int pj = 0; pj < n; pj++) {
+            P += w[pj];
+            if (P > T) {
+                sel = pj;
+                break;
+            }
+        }
+        prop[pi] = sel;
+        del[pi] = sel == pi ? 0 : dE[sel];
+    }
+    /* claims on both locations (qap.cu:1157-1199) */
+    for (int x = 0; x < n; x++) owner[x] = UINT64_MAX;
+    for (int pi = 0; pi < n; pi++) {
+        int32_t pj = prop[pi];
+        if (pj == pi) continue;
+        uint64_t key = ((uint64_t)(~((uint32_t)del[pi] ^ 0x80000000u)) << 32) | (uint32_t)pi;
+        if (key < owner[pi]) owner[pi] = key;
+        if (key < owner[pj]) owner[pj] = key;
+    }
+    /* winners: rank-2 updates from the snapshot, then the swaps (qap.cu:1203-1376) */
+    memcpy(snap, perm, sizeof(int32_t) * (size_t)n);
+    int32_t m = 0;
+    for (int pi = 0; pi < n; pi++) {
+        int32_t pj = prop[pi];
+        int32_t won = pj != pi && (uint32_t)owner[pi] == (uint32_t)pi &&
+                      (uint32_t)owner[pj] == (uint32_t)pi;
+        if (tgt) tgt[pi] = pj;
+        if (acc) acc[pi] = won;
+        if (!won) continue;
+        oracle_force_update(n, F, D, I, pi, pj, snap[pi], snap[pj]);
+        m++;
+    }
+    for (int pi = 0; pi < n; pi++) {
+        int32_t pj = prop[pi];
+        if (pj == pi || (uint32_t)owner[pi] != (uint32_t)pi || (uint32_t)owner[pj] != (uint32_t)pi)
+            continue;
+        int32_t a = snap[pi], b = snap[pj];
+        perm[pi] = b;
+        perm[pj] = a;
+        pinv[b] = pi;
+        pinv[a] = pj;
+    }
+    /* exact cost from the updated force matrix (qap.cu:1418-1457) */
+    int64_t sum = 0;
+    for (int i = 0; i < n; i++) sum += I[i * n + perm[i]];
+    *E = -sum / 2;
+    return m;
+}
+
 /* Reference floating-point roulette (CPU_PIM.cpp:212-287), Philox-driven. */
 static int32_t attempt_reffp(int32_t n, const int32_t *F, const int32_t *D, int32_t *perm,
                              int32_t *pinv, int32_t *I, int64_t *E, int32_t i, double beta2,
@@ -289,6 +364,8 @@ typedef struct {
     const double *c20;      /* per position */
     int32_t *dE;
     uint32_t *w;
+    int32_t *os_work;       /* 4n one-shot scratch */
+    uint64_t *os_owner;     /* n */
     long double *probs;
     int64_t *xE, *xbE;      /* R scratch for the exchange step */
     int32_t *xpos;
@@ -357,6 +434,18 @@ static void ladder_sweeps(ladder_t *L, int64_t m) {
         const uint32_t stream = (uint32_t)(L->ladder * R + l);
         for (int s = 0; s < par->spe; s++) {
             const uint32_t sweep = (uint32_t)(m * par->spe + s);
+            if (par->flags & ORACLE_FLAG_ONE_SHOT) {
+                int32_t acc = oracle_one_shot_step(n, P->F, P->D, rp->perm, rp->pinv, rp->I,
+                                                   &rp->E, L->beta2f[rp->pos], L->c20[rp->pos],
+                                                   par->seed, sweep, stream, L->os_work,
+                                                   L->os_owner, L->w, NULL, NULL);
+                rp->accepted += (uint64_t)acc;
+                if (acc > 0 && rp->E < rp->bestE) {
+                    rp->bestE = rp->E;
+                    memcpy(rp->best_perm, rp->perm, sizeof(int32_t) * (size_t)n);
+                }
+                continue;
+            }
             for (int i = 0; i < n; i++) {
                 uint32_t x[4];
                 ph_draw(par->seed, (uint32_t)i, sweep, stream, 0u, x);
@@ -466,6 +555,8 @@ int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_re
         L->c20 = c20;
         L->dE = malloc(sizeof(int32_t) * n);
         L->w = malloc(sizeof(uint32_t) * n);
+        L->os_work = malloc(sizeof(int32_t) * 4 * (size_t)n);
+        L->os_owner = malloc(sizeof(uint64_t) * (size_t)n);
         L->probs = malloc(sizeof(long double) * n);
         L->xE = malloc(sizeof(int64_t) * R);
         L->xbE = malloc(sizeof(int64_t) * R);
@@ -550,6 +641,8 @@ int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_re
         res->kicks += lad[t].kicks;
         free(lad[t].dE);
         free(lad[t].w);
+        free(lad[t].os_work);
+        free(lad[t].os_owner);
         free(lad[t].probs);
         free(lad[t].xE);
         free(lad[t].xbE);

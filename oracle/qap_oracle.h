@@ -45,6 +45,8 @@ enum { ORACLE_MODE_EXACT = 0, ORACLE_MODE_REFFP = 1 };
 
 enum {
     ORACLE_FLAG_INIT_PER_REPLICA = 1, /* each replica gets its own shuffle (GPU_PIM.cu:151-167) */
+    ORACLE_FLAG_ONE_SHOT = 2,         /* one-shot parallel-proposal steps instead of p-dit
+                                         sweeps (qap.cu:1037-1609), exact mode only       */
 };
 
 typedef struct {
@@ -107,6 +109,22 @@ void oracle_force_update(int32_t n, const int32_t *F, const int32_t *D, int32_t 
 int32_t oracle_attempt_exact(int32_t n, const int32_t *F, const int32_t *D, int32_t *perm,
                              int32_t *pinv, int32_t *I, int64_t *E, int32_t i, float beta2f,
                              double c20, uint32_t u, int32_t *dE, uint32_t *w);
+
+/* One one-shot step of one replica (GPU-CUPIM/perl-gpu/src/lib/qap/qap.cu:1037-1460):
+ * every location pi proposes, from the same snapshot, a swap with the facility at one
+ * location pj, drawn by the exact heat-bath rule over pj = 0..n-1 in location order
+ * (pj == pi is the stay; uniform from ctr = {pi, step, stream, 3}).  A proposal claims
+ * both of its locations with the key (~(delta ^ 2^31) << 32) | pi -- largest delta
+ * first, then smallest pi (qap.cu:1170-1199) -- and is applied when its key holds both
+ * (qap.cu:1203-1221).  Applied swaps are disjoint, so the force matrix moves by the sum
+ * of their rank-2 updates taken from the snapshot (qap.cu:1301-1376); the cost is then
+ * recomputed exactly as -1/2 sum_i I[i][p(i)] (qap.cu:1418-1457).  Scratch: work 4n,
+ * owner n, w n.  Returns the number of applied swaps; tgt (n, may be NULL) receives
+ * every location's proposal and acc (n, may be NULL) whether it was applied. */
+int32_t oracle_one_shot_step(int32_t n, const int32_t *F, const int32_t *D, int32_t *perm,
+                             int32_t *pinv, int32_t *I, int64_t *E, float beta2f, double c20,
+                             uint64_t seed, uint32_t step, uint32_t stream, int32_t *work,
+                             uint64_t *owner, uint32_t *w, int32_t *tgt, int32_t *acc);
 
 /* One exchange step of one ladder (CPU_PIM.cpp:295-344): best fold, bubble pass from
  * the hottest position, hot/cold kick.  E/bestE/pos are indexed by replica, order by

@@ -27,6 +27,7 @@ QAP_PT_E_DEVICE = -3
 QAP_PT_E_OVERFLOW = -4
 QAP_PT_E_BUSY = -5
 QAP_PT_INIT_PER_REPLICA = 1
+QAP_PT_ONE_SHOT = 2
 MAX_N = 1024
 AUTO_MIN_EXP = -30
 AUTO_STEPS = 32
@@ -200,6 +201,7 @@ class Instance:
     timeout_s: float = 0.0
     per_replica_init: bool = False
     init_perm: np.ndarray | None = None
+    one_shot: bool = False  # CUPIM's one-shot parallel-proposal steps (a different chain)
     _keep: list = field(default_factory=list, repr=False)
 
     def __post_init__(self):
@@ -227,7 +229,8 @@ class Instance:
         par.has_goal = 0 if self.goal is None else 1
         par.goal = 0 if self.goal is None else int(self.goal)
         par.timeout_s = float(self.timeout_s)
-        par.flags = QAP_PT_INIT_PER_REPLICA if self.per_replica_init else 0
+        par.flags = (QAP_PT_INIT_PER_REPLICA if self.per_replica_init else 0) | \
+            (QAP_PT_ONE_SHOT if self.one_shot else 0)
         par.init_perm = (_ptr(self.init_perm, C.c_int32) if self.init_perm is not None
                          else C.cast(None, _I32P))
         return prob, par

@@ -41,7 +41,7 @@ int auto_beta_impl(const qap_problem *prob, const qap_pt_params *base, double lo
             p.sweeps = (int64_t)QAP_PT_AUTO_MEGAS * base->sweeps_per_exchange;
             p.has_goal = 0;
             p.timeout_s = 0.0;
-            p.flags = base->flags & QAP_PT_INIT_PER_REPLICA;
+            p.flags = base->flags & (QAP_PT_INIT_PER_REPLICA | QAP_PT_ONE_SHOT);
             p.init_perm = nullptr;  // every trial starts from its own shuffle (setup(), :130-161)
             // A distinct Philox key per trial; the base seed keeps the scan reproducible.
             p.seed = base->seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(1 + i * trials + t));

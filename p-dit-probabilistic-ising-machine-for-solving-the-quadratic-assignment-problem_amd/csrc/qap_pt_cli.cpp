@@ -47,6 +47,7 @@ struct Options {
     double lower_prop = 0.1, higher_prop = 0.9;
     unsigned long long seed = 0x5eedULL;
     double tmin = 0.0, tmax = 0.0;
+    bool one_shot = false;
 };
 
 bool read_problem(const std::string &path, int &n, std::vector<int32_t> &F, std::vector<int32_t> &D) {
@@ -87,6 +88,7 @@ bool auto_select(const Options &o, int n, const std::vector<int32_t> &F, const s
     base.sweeps_per_exchange = o.numitera;
     base.seed = o.seed;
     base.kick_after = 100;
+    base.flags = o.one_shot ? QAP_PT_ONE_SHOT : 0u;  // trials run the chosen dynamics
     const auto t0 = std::chrono::steady_clock::now();
     double avg[QAP_PT_AUTO_STEPS];
     if (check(qap_pt_auto_beta(&prob, &base, o.lower_prop, o.higher_prop, avg, &beta0_ln, &ratio)))
@@ -124,7 +126,9 @@ int main(int argc, char **argv) {
         if (a == "-h" || a == "--help") {
             std::cout << "Usage: " << argv[0] << " [OPTIONS] FILE\n"
                       << "  -b -m -r -t -w -i -g -G -s -o -al -ah as the reference CPU_PIM;\n"
-                      << "  --seed S, --tmin T --tmax T (geometric temperature ladder)\n";
+                      << "  --seed S, --tmin T --tmax T (geometric temperature ladder)\n"
+                      << "  --one-shot: one-shot parallel-proposal steps (the GPU-CUPIM driver's\n"
+                      << "              dynamics) instead of p-dit sweeps; -i counts steps\n";
             return 0;
         } else if (a == "-b") {
             o.beta0_ln = std::atof(next("-b"));
@@ -145,6 +149,7 @@ int main(int argc, char **argv) {
         else if (a == "--seed") o.seed = std::strtoull(next("--seed"), nullptr, 0);
         else if (a == "--tmin") o.tmin = std::atof(next("--tmin"));
         else if (a == "--tmax") o.tmax = std::atof(next("--tmax"));
+        else if (a == "--one-shot") o.one_shot = true;
         else o.file = a;
     }
     if (o.file.empty()) {
@@ -199,6 +204,7 @@ int main(int argc, char **argv) {
     par.has_goal = o.have_goal ? 1 : 0;
     par.goal = o.goal;
     par.timeout_s = o.timeout;
+    par.flags = o.one_shot ? QAP_PT_ONE_SHOT : 0u;
     qap_pt_pool *pool = nullptr;
     if (check(qap_pt_pool_create(1, &prob, &par, &pool))) return 1;
     std::vector<int32_t> best_perm(n);

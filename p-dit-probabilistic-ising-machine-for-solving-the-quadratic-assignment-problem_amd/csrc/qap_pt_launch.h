@@ -18,6 +18,7 @@ enum IMode : int {
     IM_HBM = 2,    // int32 in HBM, updated in place
     IM_LAZY = 3,   // int32 in HBM, rank-2 updates held in LDS and applied in batches
                    // (symmetric instances; 16-bit update vectors)
+    IM_ONESHOT = 4,  // one-shot parallel-proposal steps, int32 in HBM (QAP_PT_ONE_SHOT)
 };
 
 struct SweepShape {
@@ -33,11 +34,16 @@ int sweep_lds_bytes(int n, int ld_s, int im);
 // LDS of the lazy kernel for size n, and whether it holds at least 8 pending updates
 int lazy_lds_bytes(int n);
 bool lazy_fits(int n);
+// LDS of the one-shot kernel for size n, and whether one matrix pass holds 32 update vectors
+int oneshot_lds_bytes(int n);
+bool oneshot_fits(int n);
 
 hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,
                         int lds_bytes, hipStream_t s);
 hipError_t launch_sweep_lazy(const PoolDev &P, const SweepShape &sh, const int32_t *list,
                              int nblocks, int lds_bytes, hipStream_t s);
+hipError_t launch_oneshot(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,
+                          int lds_bytes, hipStream_t s);
 hipError_t launch_exchange(const PoolDev &P, int count, hipStream_t s);
 hipError_t launch_set_limits(InstState *state, const int64_t *limits, int count, hipStream_t s);
 hipError_t launch_halt(InstState *state, int count, hipStream_t s);

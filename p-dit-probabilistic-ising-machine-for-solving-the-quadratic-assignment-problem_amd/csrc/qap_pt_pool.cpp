@@ -225,6 +225,7 @@ const char *placement_name(int im) {
     case IM_LDS32: return "lds32";
     case IM_LDS16: return "lds16";
     case IM_LAZY: return "lazy";
+    case IM_ONESHOT: return "oneshot";
     default: return "hbm";
     }
 }
@@ -319,6 +320,18 @@ int validate(const qap_problem &pb, const qap_pt_params &pr, int k) {
         }
     }
     const Bounds b = compute_bounds(pb.n, pb.F, pb.D);
+    if (pr.flags & ~(QAP_PT_INIT_PER_REPLICA | QAP_PT_ONE_SHOT))
+        return fail(QAP_PT_E_INVAL, "instance %d: unknown flags 0x%x", k, pr.flags);
+    if (pr.flags & QAP_PT_ONE_SHOT) {
+        // the one-shot kernel gives every location a lane (n <= 256 threads) and splits the
+        // 16-bit update vectors into two signed bytes for the matrix cores
+        if (pb.n > 256 || !oneshot_fits(pb.n))
+            return fail(QAP_PT_E_INVAL, "instance %d: one-shot steps need n <= 256 (n=%d)", k,
+                        pb.n);
+        if (2 * b.maxD > 32639 || 4 * b.maxF > 32639)
+            return fail(QAP_PT_E_INVAL,
+                        "instance %d: one-shot steps need 2 max|D| and 4 max|F| <= 32639", k);
+    }
     if (b.delta >= ((int64_t)1 << 30) || b.force + 8 * b.maxD * b.maxF >= ((int64_t)1 << 31))
         return fail(QAP_PT_E_OVERFLOW,
                     "instance %d: |force| bound %lld / |delta| bound %lld exceed the int32 range "
@@ -435,7 +448,11 @@ void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throu
     // lazy updates hold u (|u| <= 2 max|D|) and v (|v| <= 4 max|F|) in 16 bits, split into
     // two signed bytes for the matrix cores (the high byte of 32640..32767 would not fit)
     const bool small_uv = 2 * bnd.maxD <= 32639 && 4 * bnd.maxF <= 32639;
-    h.shape = choose_shape(n, sym, nonneg, bnd.force, sparse, small_uv, throughput, &ld_s);
+    if (pr.flags & QAP_PT_ONE_SHOT) {  // a location per thread, force matrix in HBM
+        h.shape = SweepShape{n <= 64 ? 64 : (n <= 128 ? 128 : 256), 1, IM_ONESHOT, sym};
+    } else {
+        h.shape = choose_shape(n, sym, nonneg, bnd.force, sparse, small_uv, throughput, &ld_s);
+    }
     d = InstDesc{};
     d.n = n;
     d.ld = hbm_stride(n);

@@ -980,6 +980,7 @@ hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *l
                         int lds_bytes, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (sh.im == IM_LAZY) return launch_sweep_lazy(P, sh, list, nblocks, lds_bytes, s);
+    if (sh.im == IM_ONESHOT) return launch_oneshot(P, sh, list, nblocks, lds_bytes, s);
     switch (sh.nt * 8 + sh.cpl) {
     case 64 * 8 + 1: return launch_sweep_c<64, 1>(P, sh, list, nblocks, lds_bytes, s);
     case 64 * 8 + 2: return launch_sweep_c<64, 2>(P, sh, list, nblocks, lds_bytes, s);
@@ -994,6 +995,7 @@ hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *l
 }
 
 bool sweep_shape_supported(int nt, int cpl, int im) {
+    if (im == IM_ONESHOT) return cpl == 1 && (nt == 64 || nt == 128 || nt == 256);
     if (im == IM_LAZY && nt == 512 && (cpl == 1 || cpl == 2)) return true;
     switch (nt * 8 + cpl) {
     case 64 * 8 + 1: case 64 * 8 + 2: case 64 * 8 + 4: case 128 * 8 + 1: case 128 * 8 + 2:
@@ -1003,6 +1005,7 @@ bool sweep_shape_supported(int nt, int cpl, int im) {
 }
 
 int sweep_lds_bytes(int n, int ld_s, int im) {
+    if (im == IM_ONESHOT) return oneshot_lds_bytes(n);
     return im == IM_LAZY ? lazy_lds_bytes(n) : 4 * sweep_lds_words(n, ld_s, im);
 }
 

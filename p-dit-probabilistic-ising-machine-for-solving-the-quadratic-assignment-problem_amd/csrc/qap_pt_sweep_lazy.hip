@@ -23,6 +23,7 @@
 // trajectories are identical to the in-LDS and in-HBM kernels.
 #include "qap_pt_device.h"
 #include "qap_pt_launch.h"
+#include "qap_pt_mfma_update.h"
 #include "qap_pt_sweep_common.h"
 
 namespace qap {
@@ -31,8 +32,6 @@ namespace {
 
 constexpr int kLazyRed = 160;  // reduction slots (same use as the other sweep kernel)
 constexpr int kLazyMaxK = 32;
-constexpr int kFlushTiles = 4;  // 16-column tiles a wave updates per pass over the rows
-constexpr int kFlushDepth = 4;  // row tiles of a flush in flight per wave
 // p, pinv, cls, rnd, dg, drow x2, frow x2, rowI x2, colA x2 -- round8(n) words each
 constexpr int kLazyArrays = 13;
 
@@ -55,8 +54,6 @@ __host__ __device__ inline LazyLayout lazy_layout(int n, int lds_bytes) {
     return LazyLayout{8 * (kg > 0 ? kg : 0), 8 * (sg > 0 ? sg : 0)};
 }
 
-using v4h = int __attribute__((ext_vector_type(4)));  // 8 packed int16
-
 // sum_k x[k] y[k] over packed signed 16-bit entries, 8 per 128-bit word: each half is
 // sign-extended and multiplied in 32-bit integer arithmetic (exact)
 __device__ __forceinline__ int32_t dot8(v4h x, v4h y, int32_t acc) {
@@ -67,22 +64,6 @@ __device__ __forceinline__ int32_t dot8(v4h x, v4h y, int32_t acc) {
         acc += xl * yl + xh * yh;
     }
     return acc;
-}
-
-// Signed byte planes of 8 packed int16 entries x = lo + 256 hi: lo is the low byte read
-// as int8, hi = (x - lo) / 256 = high byte + bit 7 of the low byte (entries need
-// |x| <= 32639 so that hi stays within int8).  Bytes keep the entry order.
-__device__ __forceinline__ void split_bytes(v4h x, long &lo, long &hi) {
-    uint32_t l[2], h[2];
-#pragma unroll
-    for (int w = 0; w < 2; ++w) {
-        l[w] = __builtin_amdgcn_perm((uint32_t)x[2 * w + 1], (uint32_t)x[2 * w], 0x06040200u);
-        const uint32_t hb = __builtin_amdgcn_perm((uint32_t)x[2 * w + 1], (uint32_t)x[2 * w], 0x07050301u);
-        const uint32_t carry = (l[w] >> 7) & 0x01010101u;
-        h[w] = ((hb & 0x7f7f7f7fu) + carry) ^ (hb & 0x80808080u);  // bytewise add, no carries
-    }
-    lo = (long)(((uint64_t)l[1] << 32) | l[0]);
-    hi = (long)(((uint64_t)h[1] << 32) | h[0]);
 }
 
 }  // namespace
@@ -176,92 +157,10 @@ __global__ __launch_bounds__(NT) void sweep_lazy_kernel(PoolDev P, const int32_t
     bool stale = false;  // a flush made the prefetched I_base row / column out of date
 
     // I_base += U V^T over the whole matrix on the matrix cores, then no pending update is
-    // left.  Each wave takes chunks of kFlushTiles 16-column tiles and computes the
-    // transposed tiles (V U^T) with v_mfma_i32_16x16x32_i8, whose accumulator layout
-    // (lane l: row l % 16, columns 4 (l / 16) + 0..3 of the I tile) makes every lane's
-    // share of a tile one 16-byte load and store.  A lane's operand bytes are pending
-    // entries 8 (l / 16) + 0..7 of one row of V (column l % 16 of the tile) or U (row
-    // l % 16), the same entries for both operands; entries at or beyond K are zero.  Row
-    // tiles are loaded kFlushDepth - 1 tiles ahead of their update.
+    // left
     auto flush = [&]() {
         if (K == 0) return;
-        const int groups = K / 8 + ((K & 7) != 0);
-        const int li = lane & 15, kg = lane >> 4;
-        const int ntile = (n + 15) / 16;
-        const int nchunk = (ntile + kFlushTiles - 1) / kFlushTiles;
-        for (int ch = wid; ch < nchunk; ch += NW) {
-            const int cb = ch * kFlushTiles * 16;  // first column of the chunk
-            long a0[kFlushTiles], a1[kFlushTiles];
-            bool any_hi = false;
-#pragma unroll
-            for (int t = 0; t < kFlushTiles; ++t) {
-                const int c = cb + t * 16 + li;
-                v4h w = v4h{0, 0, 0, 0};
-                if (c < n && kg < groups) w = reinterpret_cast<const v4h *>(V_s + (size_t)c * kpad)[kg];
-                split_bytes(w, a0[t], a1[t]);
-                any_hi |= a1[t] != 0;
-            }
-            const bool vh = __ballot(any_hi) != 0;  // some V entry needs its high byte
-
-            v4i cur[kFlushDepth][kFlushTiles];
-            // I rows x0 + (l % 16), columns 4 (l / 16) + 0..3 of each tile of the chunk.  The
-            // loads are unconditional (lanes past the matrix read its last row / 4 columns
-            // and store nothing), so no branch keeps them from staying in flight.
-            const int clast = (n - 1) & ~3;
-            auto load_rows = [&](v4i(&dst)[kFlushTiles], int x0) {
-                const int xr = min(x0 + li, n - 1);
-#pragma unroll
-                for (int t = 0; t < kFlushTiles; ++t) {
-                    const int cr = min(cb + t * 16 + 4 * kg, clast);
-                    dst[t] = *reinterpret_cast<const v4i *>(Ig + (int64_t)xr * ld + cr);
-                }
-            };
-            auto update_rows = [&](const v4i(&src)[kFlushTiles], int x0) {
-                if (x0 >= n) return;
-                const int x = x0 + li;
-                v4h w = v4h{0, 0, 0, 0};
-                if (x < n && kg < groups) w = reinterpret_cast<const v4h *>(U_s + (size_t)x * kpad)[kg];
-                long b0, b1;
-                split_bytes(w, b0, b1);
-                const bool uh = __ballot(b1 != 0) != 0;  // some U entry needs its high byte
-                const v4i z = v4i{0, 0, 0, 0};
-#pragma unroll
-                for (int t = 0; t < kFlushTiles; ++t) {
-                    const int c = cb + t * 16 + 4 * kg;
-                    if (cb + t * 16 >= n) break;
-                    // V U^T = a0 b0 + 2^8 (a0 b1 + a1 b0) + 2^16 a1 b1, summed mod 2^32
-                    const v4i s0 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a0[t], b0, z, 0, 0, 0);
-                    v4i s1 = z, s2 = z;
-                    if (uh) s1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a0[t], b1, s1, 0, 0, 0);
-                    if (vh) {
-                        s1 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1[t], b0, s1, 0, 0, 0);
-                        if (uh) s2 = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1[t], b1, s2, 0, 0, 0);
-                    }
-                    v4i r;
-#pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        r[v] = (int32_t)((uint32_t)src[t][v] + (uint32_t)s0[v] + ((uint32_t)s1[v] << 8) +
-                                         ((uint32_t)s2[v] << 16));
-                    if (x < n && c < n) *reinterpret_cast<v4i *>(Ig + (int64_t)x * ld + c) = r;
-                }
-            };
-            // scheduling barriers keep the loads in row-tile order, so waiting for the
-            // oldest tile does not wait for the younger ones
-#pragma unroll
-            for (int dd = 0; dd < kFlushDepth - 1; ++dd) {
-                load_rows(cur[dd], 16 * dd);
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            for (int x0 = 0; x0 < n; x0 += 16 * kFlushDepth) {
-#pragma unroll
-                for (int dd = 0; dd < kFlushDepth; ++dd) {
-                    load_rows(cur[(dd + kFlushDepth - 1) % kFlushDepth], x0 + 16 * (dd + kFlushDepth - 1));
-                    __builtin_amdgcn_sched_barrier(0);
-                    update_rows(cur[dd], x0 + 16 * dd);
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-            }
-        }
+        mfma_rank_update<NW, 1>(Ig, ld, n, U_s, V_s, kpad, K, lane, wid);
         __syncthreads();  // the matrix writes reach every wave before it is read again
         zero_pending();
         K = 0;

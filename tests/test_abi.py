@@ -54,6 +54,26 @@ def test_host_validation_errors(qpt):
     _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(**dict(ok, F=Fz, D=Fz)))
 
 
+def test_one_shot_limits_are_checked_on_the_host(qpt):
+    """One-shot steps need n <= 256 and update vectors that split into two signed bytes
+    (2 max|D|, 4 max|F| <= 32639); anything else is refused before any device work."""
+    F, D = qpt.instances.random_instance(2, 257)
+    _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(F=F, D=D, beta_ln=np.array([0.1]), sweeps=1,
+                                                 one_shot=True))
+    F, D = qpt.instances.random_instance(3, 8)
+    F = F * 1000  # 4 max|F| = 36000
+    _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(F=F, D=D, beta_ln=np.array([0.1]), sweeps=1,
+                                                 one_shot=True))
+    import ctypes as C
+    F, D = qpt.instances.random_instance(4, 8)
+    inst = qpt.Instance(F=F, D=D, beta_ln=np.array([0.1]), sweeps=1)
+    prob, par = inst.c_structs()
+    par.flags = 8  # no such flag
+    out = C.c_void_p()
+    rc = qpt.load_library().qap_pt_pool_create(1, C.byref(prob), C.byref(par), C.byref(out))
+    assert rc == qpt.QAP_PT_E_INVAL
+
+
 def test_no_device_is_an_error_not_a_fallback(qpt):
     """Without a visible GPU a valid request fails with QAP_PT_E_DEVICE: there is no
     CPU path inside the product library."""

@@ -16,14 +16,15 @@ pytestmark = pytest.mark.gpu
 
 
 def _run_both(qpt, oracle, F, D, betas, sweeps, T=1, spe=1, seed=7, kick_after=100, goal=None,
-              per_replica_init=False, env=None, init_perm=None):
+              per_replica_init=False, env=None, init_perm=None, one_shot=False):
     env = env or {}
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
         inst = qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=sweeps, T=T, sweeps_per_exchange=spe,
                             seed=seed, kick_after=kick_after, goal=goal,
-                            per_replica_init=per_replica_init, init_perm=init_perm)
+                            per_replica_init=per_replica_init, init_perm=init_perm,
+                            one_shot=one_shot)
         with qpt.Pool([inst]) as pool:
             pool.run(-1)
             res = pool.result(0)
@@ -36,7 +37,8 @@ def _run_both(qpt, oracle, F, D, betas, sweeps, T=1, spe=1, seed=7, kick_after=1
             else:
                 os.environ[k] = v
     ref = oracle.solve(F, D, betas, sweeps, T=T, spe=spe, seed=seed, kick_after=kick_after,
-                       goal=goal, per_replica_init=per_replica_init, init_perm=init_perm)
+                       goal=goal, per_replica_init=per_replica_init, init_perm=init_perm,
+                       one_shot=one_shot)
     return res, st, (fm, cm), ref
 
 
@@ -117,6 +119,25 @@ CASES = {
                            {"env": {"QAP_PT_SHAPE": "512,1"}}),
     "lazy_n700_512x2": (lambda q: q.instances.random_instance(34, 700), 2, 1,
                         {"env": {"QAP_PT_IMODE": "3", "QAP_PT_SHAPE": "512,2"}}),
+    # one-shot parallel-proposal steps (qap.cu:1037-1609): 64 / 128 / 256 threads
+    "os_sym_n16_R4": (lambda q: q.instances.random_instance(11, 16), 4, 40, {"one_shot": True}),
+    "os_c1_n64_R16": (lambda q: q.instances.flat_c1(), 16, 12, {"one_shot": True}),
+    "os_n37_T2_spe3": (lambda q: q.instances.random_instance(35, 37), 4, 12,
+                       {"one_shot": True, "T": 2, "spe": 3}),
+    "os_sym_n100_R8": (lambda q: q.instances.mixed_instance(5, 100, True), 8, 6,
+                       {"one_shot": True}),
+    "os_sparse_n150_R4": (lambda q: q.instances.mixed_instance(6, 150, False), 4, 4,
+                          {"one_shot": True}),
+    "os_asym_diag_n24_T2": (lambda q: q.instances.random_instance(12, 24, symmetric=False,
+                                                                 zero_diag=False), 6, 20,
+                            {"one_shot": True, "T": 2, "spe": 2}),
+    "os_asym_n70_R5": (lambda q: q.instances.random_instance(13, 70, 0, 50, symmetric=False),
+                       5, 6, {"one_shot": True}),
+    "os_asym_n200_R2": (lambda q: q.instances.random_instance(36, 200, 0, 30, symmetric=False),
+                        2, 2, {"one_shot": True}),
+    "os_grey_n256_R3": (lambda q: q.instances.grey_c2(), 3, 3, {"one_shot": True}),
+    "os_n256_hot_R2": (lambda q: q.instances.mixed_instance(37, 256, True), 2, 2,
+                       {"one_shot": True}),
 }
 
 
@@ -168,6 +189,27 @@ def test_batch_matches_single_and_oracle(qpt, oracle, gpu):
         assert batch[k]["best_cost"] == single["best_cost"] == ref["best_cost"]
         np.testing.assert_array_equal(batch[k]["best_perm"], ref["best_perm"])
         assert batch[k]["accepted"] == ref["accepted"]
+
+
+def test_batch_mixes_one_shot_and_sweep_instances(qpt, oracle, gpu):
+    """One-shot and p-dit instances share a pool (separate kernels in the same launch
+    sequence); each ends where its own oracle run does."""
+    insts = []
+    for k, (n, one_shot) in enumerate([(40, True), (40, False), (100, True), (23, True),
+                                       (180, True)]):
+        F, D = qpt.instances.mixed_instance(120 + k, n, dense=(k % 2 == 0))
+        betas = qpt.instances.geometric_ladder(3 + k, 1.0, 100.0) / max(1.0, n / 16.0)
+        insts.append(qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=3 * (k + 1), seed=70 + k,
+                                  one_shot=one_shot))
+    batch = qpt.solve_batch(insts)
+    for k, inst in enumerate(insts):
+        ref = oracle.solve(inst.F, inst.D, inst.beta_ln, inst.sweeps, seed=inst.seed,
+                           one_shot=inst.one_shot)
+        print(f"instance {k}: n={inst.n} one_shot={inst.one_shot}", flush=True)
+        assert batch[k]["best_cost"] == ref["best_cost"]
+        np.testing.assert_array_equal(batch[k]["best_perm"], ref["best_perm"])
+        assert batch[k]["accepted"] == ref["accepted"]
+        assert batch[k]["attempts"] == ref["attempts"]
 
 
 def test_timed_mode_same_trajectory(qpt, oracle, gpu):
