@@ -190,6 +190,65 @@ def batch_vs_plain(pkg, args):
     }
 
 
+def dynamics_tts(pkg, args):
+    """Time-to-target of the two chains on --workload: the p-dit chain for --tts-sweeps
+    sweeps in --tts-chunks pieces, then the one-shot chain in pieces of the same number
+    of steps until it has used as much device time (at most 20x the pieces).  Target: the
+    p-dit chain's best cost at the full sweep budget + 1 % (that chain is bit-identical to
+    the CPU restatement, so this is also the CPU run's best at equal sweeps)."""
+    insts, desc, _ = workload(pkg, args.workload)
+    sweeps, chunks = args.tts_sweeps, args.tts_chunks
+
+    def run(one_shot, budget_s=None):
+        for x in insts:
+            x.one_shot = one_shot
+        pool = pkg.Pool(insts)
+        step = sweeps // chunks
+        clock, traj, att = 0.0, [], 0
+        for c in range(chunks * (20 if budget_s else 1)):
+            t0 = time.perf_counter()
+            pool.run(step)
+            clock += time.perf_counter() - t0
+            traj.append((clock, min(int(b) for b in pool.best_costs())))
+            if budget_s and clock >= budget_s:
+                break
+        att = sum(pool.result(k)["attempts"] for k in range(len(insts)))
+        fm, cm = pool.verify()
+        pool.close()
+        return traj, att, fm + cm
+
+    tp, ap, vp = run(False)
+    print(f"# p-dit: {sweeps} sweeps, {tp[-1][0]:.2f} s, best {tp[-1][1]}", flush=True)
+    to, ao, vo = run(True, budget_s=tp[-1][0])
+    print(f"# one-shot: {len(to) * (sweeps // chunks)} steps, {to[-1][0]:.2f} s, best {to[-1][1]}",
+          flush=True)
+    final = tp[-1][1]
+    target = final + abs(final) // 100
+
+    def tts(traj, goal=target):
+        return next((t for t, b in traj if b <= goal), None)
+
+    return {
+        "metric": "time-to-target (one-shot vs p-dit chain)",
+        "value": tts(to),
+        "unit": "s",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": dict(desc, sweeps=sweeps, pieces=chunks),
+        "target": target,
+        "exact_target": final,
+        "pdit": {"tts_s": tts(tp), "tts_exact_s": tts(tp, final), "seconds": tp[-1][0],
+                 "attempts_per_s": ap / tp[-1][0],
+                 "best": tp[-1][1], "verify_mismatches": vp},
+        "one_shot": {"tts_s": tts(to), "tts_exact_s": tts(to, final), "seconds": to[-1][0],
+                     "attempts_per_s": ao / to[-1][0],
+                     "steps": len(to) * (sweeps // chunks), "best": to[-1][1],
+                     "verify_mismatches": vo},
+    }
+
+
 def served(pkg, args):
     """Poisson request streams through the serving loop (qap_srv_*): n log-uniform in
     [20,256], flow/distance as the batched workload, 32 replicas, a fixed sweep budget;
@@ -277,12 +336,17 @@ def main():
     ap.add_argument("--workload", default="grey256",
                     choices=["grey256", "flat64", "batch512", "large512", "large1024"])
     ap.add_argument("--sweeps-per-step", type=int, default=0)
+    ap.add_argument("--dynamics", default="pdit", choices=["pdit", "one-shot"],
+                    help="pdit: sequential p-dit sweeps (CPU_PIM's chain, the default); "
+                         "one-shot: parallel-proposal steps (GPU-CUPIM's driver), a step "
+                         "counts as a sweep of n attempts")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--mode", default="throughput",
-                    choices=["throughput", "batch-vs-plain", "served"],
+                    choices=["throughput", "batch-vs-plain", "served", "dynamics"],
                     help="throughput: the one-line bench of --workload; batch-vs-plain and "
-                         "served: time-to-target / latency percentiles against the plain path")
+                         "served: time-to-target / latency percentiles against the plain path; "
+                         "dynamics: time-to-target of the one-shot chain against the p-dit chain")
     ap.add_argument("--tts-sweeps", type=int, default=10000)
     ap.add_argument("--tts-chunks", type=int, default=50)
     ap.add_argument("--plain-sample", type=int, default=16)
@@ -328,7 +392,14 @@ def main():
     if args.mode == "served":
         print(json.dumps(served(pkg, args)))
         return
+    if args.mode == "dynamics":
+        print(json.dumps(dynamics_tts(pkg, args)))
+        return
     insts, desc, cpu_spec = workload(pkg, args.workload)
+    if args.dynamics == "one-shot":
+        for x in insts:
+            x.one_shot = True
+        desc = dict(desc, dynamics="one-shot parallel proposals (a step = n proposals)")
     sps = args.sweeps_per_step or {"grey256": 100, "flat64": 200, "batch512": 10,
                                    "large512": 10, "large1024": 4}[args.workload]
 
@@ -355,7 +426,8 @@ def main():
     achieved = alg / max(launches, 1) / avg_launch_s / 1e9
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.txt")
+    suffix = "_oneshot" if args.dynamics == "one-shot" else ""
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}{suffix}.txt")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
@@ -388,7 +460,7 @@ def main():
         "verify_mismatches": fm + cm,
         "best_cost": min(a["best_cost"] for a in after),
     }
-    if not args.no_cpu:
+    if not args.no_cpu and args.dynamics == "pdit":
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(pkg, insts, cpu_spec, threads)
     print(json.dumps(out))
