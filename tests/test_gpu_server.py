@@ -94,3 +94,20 @@ def test_banks_mixed_classes_goal_flight_and_empty_run(qpt, gpu):
         assert got[k]["sweeps_done"] == alone[k]["sweeps_done"]
         assert got[k]["exchanges"] == alone[k]["exchanges"]
     assert alone[-4]["sweeps_done"] < 40          # the goal stopped it early
+
+
+def test_one_shot_requests_run_beside_sweep_requests(qpt, oracle, gpu):
+    """One-shot requests (pools of their own) and p-dit requests (slot banks) in one stream;
+    every result equals the oracle's run of its own dynamics."""
+    reqs = _requests(qpt, 6, seed0=700)
+    for k in (1, 3, 4):
+        reqs[k].one_shot = True
+    with qpt.Server(max_pools=4, max_batch=4) as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        got = [srv.wait(t, timeout_s=60.0) for t in tickets]
+    for k, r in enumerate(reqs):
+        assert got[k] is not None and got[k]["status"] == qpt.QAP_SRV_DONE, k
+        ref = oracle.solve(r.F, r.D, r.beta_ln, r.sweeps, seed=r.seed, one_shot=r.one_shot)
+        assert got[k]["best_cost"] == ref["best_cost"], k
+        np.testing.assert_array_equal(got[k]["best_perm"], ref["best_perm"])
+        assert got[k]["accepted"] == ref["accepted"]
