@@ -17,7 +17,7 @@ namespace qap {
 using v4h = int __attribute__((ext_vector_type(4)));  // 8 packed int16
 
 constexpr int kUpdTiles = 4;  // 16-column tiles a wave updates per pass over the rows
-constexpr int kUpdDepth = 4;  // row tiles in flight per wave
+constexpr int kUpdDepth = 4;  // row tiles in flight per wave (default)
 
 // Signed byte planes of 8 packed int16 entries x = lo + 256 hi: lo is the low byte read
 // as int8, hi = (x - lo) / 256 = high byte + bit 7 of the low byte.  Bytes keep the
@@ -45,10 +45,10 @@ __device__ __forceinline__ void split_bytes(v4h x, long &lo, long &hi) {
 // columns 4 (l / 16) + 0..3 of the I tile) makes every lane's share of a tile one 16-byte
 // load and store.  A lane's operand bytes for 32-entry chunk kc are entries
 // 32 kc + 8 (l / 16) + 0..7 of one row of V (column l % 16 of the tile) or of U (row
-// l % 16).  Row tiles are loaded kUpdDepth - 1 tiles ahead of their update; the loads
+// l % 16).  Row tiles are loaded DEPTH - 1 tiles ahead of their update; the loads
 // are unconditional (lanes past the matrix read its last row / 4 columns and store
 // nothing), so no branch keeps them from staying in flight.
-template <int NW, int KCH>
+template <int NW, int KCH, int DEPTH = kUpdDepth>
 __device__ __forceinline__ void mfma_rank_update(int32_t *Ig, int ld, int n, const int16_t *U_s,
                                                  const int16_t *V_s, int kst, int K, int lane,
                                                  int wid) {
@@ -74,7 +74,7 @@ __device__ __forceinline__ void mfma_rank_update(int32_t *Ig, int ld, int n, con
             }
         const bool vh = __ballot(any_hi) != 0;  // some V entry needs its high byte
 
-        v4i cur[kUpdDepth][kUpdTiles];
+        v4i cur[DEPTH][kUpdTiles];
         auto load_rows = [&](v4i(&dst)[kUpdTiles], int x0) {
             const int xr = min(x0 + li, n - 1);
 #pragma unroll
@@ -124,14 +124,14 @@ __device__ __forceinline__ void mfma_rank_update(int32_t *Ig, int ld, int n, con
         // scheduling barriers keep the loads in row-tile order, so waiting for the
         // oldest tile does not wait for the younger ones
 #pragma unroll
-        for (int dd = 0; dd < kUpdDepth - 1; ++dd) {
+        for (int dd = 0; dd < DEPTH - 1; ++dd) {
             load_rows(cur[dd], 16 * dd);
             __builtin_amdgcn_sched_barrier(0);
         }
-        for (int x0 = 0; x0 < n; x0 += 16 * kUpdDepth) {
+        for (int x0 = 0; x0 < n; x0 += 16 * DEPTH) {
 #pragma unroll
-            for (int dd = 0; dd < kUpdDepth; ++dd) {
-                load_rows(cur[(dd + kUpdDepth - 1) % kUpdDepth], x0 + 16 * (dd + kUpdDepth - 1));
+            for (int dd = 0; dd < DEPTH; ++dd) {
+                load_rows(cur[(dd + DEPTH - 1) % DEPTH], x0 + 16 * (dd + DEPTH - 1));
                 __builtin_amdgcn_sched_barrier(0);
                 update_rows(cur[dd], x0 + 16 * dd);
                 __builtin_amdgcn_sched_barrier(0);

@@ -32,7 +32,8 @@ namespace {
 
 constexpr int kOsRed = 64;     // reduction slots
 constexpr int kOsMaxKCH = 4;   // update vectors per matrix pass: at most 4 x 32
-constexpr int kOsBatch = 4;    // locations whose loads a wave issues together
+constexpr int kPanel = 64;     // facilities (force-matrix columns) staged in LDS at a time
+constexpr int kPanStride = kPanel + 1;  // LDS row stride of a panel: column reads conflict-free
 // p, pinv, dg, rnd, prop, del, list -- round8(n) words each -- and owner (2 words each)
 constexpr int kOsArrays = 9;
 
@@ -66,13 +67,20 @@ int oneshot_lds_bytes(int n) {
     return want < kMaxLdsBytes ? want : kMaxLdsBytes;
 }
 
-bool oneshot_fits(int n) { return os_layout(n, oneshot_lds_bytes(n)).kcap >= 32; }
+bool oneshot_fits(int n) {
+    const OsLayout l = os_layout(n, oneshot_lds_bytes(n));
+    return l.kcap >= 32 && l.kst >= kPanStride;  // a column panel fits the U / V region
+}
 
-template <int NT>
+template <int NT, int CPL>
 __global__ __launch_bounds__(NT) void oneshot_kernel(PoolDev P, const int32_t *__restrict__ rep_list,
                                                      int lds_bytes) {
-    constexpr int NW = NT / kWave;
-    constexpr int CPL = NW;  // candidates per lane: n <= NT
+    constexpr int NW = NT / kWave;  // candidates pj = lane + 64 q, q < CPL: n <= 64 CPL
+    // 32-entry chunks of update vectors per matrix pass: eight-wave groups have half the
+    // registers per lane, so they take half as many per pass
+    constexpr int KCH = NT >= 512 ? kOsMaxKCH / 2 : kOsMaxKCH;
+    // locations whose loads a wave issues together (the same 16 per workgroup either way)
+    constexpr int kOsBatch = NT >= 512 ? 2 : 4;
     extern __shared__ __align__(16) int32_t smem[];
 
     const int g = rep_list[blockIdx.x];
@@ -83,7 +91,9 @@ __global__ __launch_bounds__(NT) void oneshot_kernel(PoolDev P, const int32_t *_
     const int n = d.n, ld = d.ld, n4 = round8(n);
     const int local = g - d.rep_base;
     const OsLayout lay = os_layout(n, lds_bytes);
-    if (lay.kcap < 32 || n > NT) return;  // the host launches only shapes that fit
+    // the host launches only shapes that fit: a location per thread, a candidate slot per
+    // lane and location, and a panel inside the update-vector region
+    if (lay.kcap < 32 || n > NT || n > kWave * CPL || lay.kst < kPanStride) return;
 
     int32_t *p_s = smem;
     int32_t *pinv_s = p_s + n4;
@@ -96,6 +106,7 @@ __global__ __launch_bounds__(NT) void oneshot_kernel(PoolDev P, const int32_t *_
     int32_t *red_s = reinterpret_cast<int32_t *>(own_s + n4);
     int16_t *U_s = reinterpret_cast<int16_t *>(red_s + kOsRed);  // [n4][kst]
     int16_t *V_s = U_s + (size_t)n4 * lay.kst;                    // [n4][kst]
+    int32_t *pan_s = reinterpret_cast<int32_t *>(U_s);  // [n][kPanStride] during proposals
 
     const int32_t *__restrict__ F = P.F + d.mat_off;
     const int32_t *__restrict__ D = P.D + d.mat_off;
@@ -136,13 +147,31 @@ __global__ __launch_bounds__(NT) void oneshot_kernel(PoolDev P, const int32_t *_
         }
         lds_sync<NW>();
 
-        // ---- proposals: wave wid takes locations wid, wid + NW, ...; the loads of
-        // kOsBatch locations are issued before any of their picks
-        for (int pb = wid * kOsBatch; pb < n; pb += NW * kOsBatch) {
+        // ---- proposals, one panel of kPanel facilities at a time: the panel's columns of I
+        // are staged in LDS with coalesced 16-byte loads, then the locations holding
+        // those facilities (pinv of the panel) take their column I[.][a] from LDS.  Wave
+        // wid takes the panel's locations wid * kOsBatch, ...; the loads of kOsBatch
+        // locations are issued before any of their picks.
+        for (int f0 = 0; f0 < n; f0 += kPanel) {
+            const int fw = min(kPanel, n - f0);
+            const int ncg = (fw + 3) / 4;  // 4-column groups of the panel
+            for (int e = tid; e < n * (kPanel / 4); e += NT) {
+                const int x = e / (kPanel / 4), cg = e % (kPanel / 4);
+                if (cg < ncg) {  // columns f0 + 4 cg .. + 3 < round4(n) <= ld: inside row x
+                    const v4i v = *reinterpret_cast<const v4i *>(Ig + (int64_t)x * ld + f0 + 4 * cg);
+                    int32_t *dst = pan_s + x * kPanStride + 4 * cg;
+                    dst[0] = v.x;
+                    dst[1] = v.y;
+                    dst[2] = v.z;
+                    dst[3] = v.w;
+                }
+            }
+            lds_sync<NW>();
+        for (int pb = wid * kOsBatch; pb < fw; pb += NW * kOsBatch) {
             int32_t dE[kOsBatch][CPL];
 #pragma unroll
             for (int t = 0; t < kOsBatch; ++t) {
-                const int pi = min(pb + t, n - 1);
+                const int pi = pinv_s[f0 + min(pb + t, fw - 1)];
                 const int a = p_s[pi];
                 const int32_t *rowI = Ig + (int64_t)pi * ld;
 #pragma unroll
@@ -150,7 +179,7 @@ __global__ __launch_bounds__(NT) void oneshot_kernel(PoolDev P, const int32_t *_
                     const int pj = min(lane + q * kWave, n - 1);
                     const int b = p_s[pj];
                     const int32_t r = rowI[b];                       // I[pi][b]
-                    const int32_t c = Ig[(int64_t)pj * ld + a];      // I[pj][a]
+                    const int32_t c = pan_s[pj * kPanStride + (a - f0)];  // I[pj][a]
                     int32_t df;
                     if (sym) {
                         df = 4 * D[(int64_t)pi * n + pj] * F[(int64_t)a * n + b];
@@ -164,29 +193,46 @@ __global__ __launch_bounds__(NT) void oneshot_kernel(PoolDev P, const int32_t *_
                     dE[t][q] = r + c - dg_s[pi] - dg_s[pj] - df;
                 }
             }
+            // the kOsBatch picks run phase by phase, so their wave scans (dependent DPP
+            // chains) interleave; a slot past the last location takes no part
+            int32_t maxd[kOsBatch];
 #pragma unroll
             for (int t = 0; t < kOsBatch; ++t) {
-                const int pi = pb + t;
-                if (pi >= n) break;
                 int lmax = INT32_MIN;
 #pragma unroll
                 for (int q = 0; q < CPL; ++q)
                     if (lane + q * kWave < n) lmax = max(lmax, dE[t][q]);
-                const int32_t maxd = lane63(wave_incl_max(lmax));  // >= 0: the stay is a candidate
-                const int32_t cut = heat_cutoff(maxd, c20);
-                uint32_t w[CPL], ex[CPL], S = 0;
+                maxd[t] = wave_incl_max(lmax);
+            }
+            uint32_t w[kOsBatch][CPL], incl[kOsBatch][CPL];
+#pragma unroll
+            for (int t = 0; t < kOsBatch; ++t) {
+                maxd[t] = lane63(maxd[t]);  // >= 0: the stay is a candidate
+                const int32_t cut = heat_cutoff(maxd[t], c20);
 #pragma unroll
                 for (int q = 0; q < CPL; ++q) {
                     const bool live = lane + q * kWave < n && dE[t][q] > cut;
-                    w[q] = live ? heat_weight(beta2f, dE[t][q] - maxd) : 0u;
-                    const uint32_t incl = wave_incl_sum(w[q]);
-                    ex[q] = S + incl - w[q];
-                    S += (uint32_t)lane63((int)incl);
+                    w[t][q] = live ? heat_weight(beta2f, dE[t][q] - maxd[t]) : 0u;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < kOsBatch; ++t)
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) incl[t][q] = wave_incl_sum(w[t][q]);
+#pragma unroll
+            for (int t = 0; t < kOsBatch; ++t) {
+                if (pb + t >= fw) break;
+                const int pi = pinv_s[f0 + pb + t];
+                uint32_t S = 0, ex[CPL];
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    ex[q] = S + incl[t][q] - w[t][q];
+                    S += (uint32_t)lane63((int)incl[t][q]);
                 }
                 const uint32_t thr = (uint32_t)(((uint64_t)rnd_s[pi] * (uint64_t)S) >> 32);
 #pragma unroll
                 for (int q = 0; q < CPL; ++q) {
-                    if (w[q] > 0 && ex[q] <= thr && thr < ex[q] + w[q]) {
+                    if (w[t][q] > 0 && ex[q] <= thr && thr < ex[q] + w[t][q]) {
                         const int pj = lane + q * kWave;
                         prop_s[pi] = pj;
                         del_s[pi] = pj == pi ? 0 : dE[t][q];
@@ -198,7 +244,8 @@ __global__ __launch_bounds__(NT) void oneshot_kernel(PoolDev P, const int32_t *_
                 }
             }
         }
-        lds_sync<NW>();
+            lds_sync<NW>();  // the panel is read; proposals of its locations are in LDS
+        }
 
         // ---- claims on both locations of every moving proposal
         for (int pi = tid; pi < n; pi += NT) {
@@ -235,7 +282,7 @@ __global__ __launch_bounds__(NT) void oneshot_kernel(PoolDev P, const int32_t *_
         // ---- sum of the winners' rank-2 updates (from the snapshot), in passes of at
         // most kcap update vectors
         const int per = sym ? 1 : 2;  // update vectors per swap
-        const int spp = lay.kcap / per;
+        const int spp = min(lay.kcap, 32 * KCH) / per;
         for (int s0 = 0; s0 < m; s0 += spp) {
             const int ms = min(spp, m - s0), K = ms * per, K8 = round8(K);
             for (int e = tid; e < ms * n; e += NT) {
@@ -267,7 +314,8 @@ __global__ __launch_bounds__(NT) void oneshot_kernel(PoolDev P, const int32_t *_
                 V_s[(size_t)x * lay.kst + k] = 0;
             }
             lds_sync<NW>();
-            mfma_rank_update<NW, kOsMaxKCH>(Ig, ld, n, U_s, V_s, lay.kst, K, lane, wid);
+            mfma_rank_update<NW, KCH, NT >= 512 ? 2 : kUpdDepth>(Ig, ld, n, U_s, V_s, lay.kst, K,
+                                                                lane, wid);
             __syncthreads();  // matrix writes visible, U / V free for the next pass
         }
 
@@ -319,10 +367,10 @@ __global__ __launch_bounds__(NT) void oneshot_kernel(PoolDev P, const int32_t *_
     }
 }
 
-template <int NT>
+template <int NT, int CPL>
 hipError_t launch_oneshot_t(const PoolDev &P, const int32_t *list, int nblocks, int lds_bytes,
                             hipStream_t s) {
-    auto k = oneshot_kernel<NT>;
+    auto k = oneshot_kernel<NT, CPL>;
     static bool attr_done = false;
     if (!attr_done) {
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -335,10 +383,13 @@ hipError_t launch_oneshot_t(const PoolDev &P, const int32_t *list, int nblocks, 
 
 hipError_t launch_oneshot(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,
                           int lds_bytes, hipStream_t s) {
-    switch (sh.nt) {
-    case 64: return launch_oneshot_t<64>(P, list, nblocks, lds_bytes, s);
-    case 128: return launch_oneshot_t<128>(P, list, nblocks, lds_bytes, s);
-    case 256: return launch_oneshot_t<256>(P, list, nblocks, lds_bytes, s);
+    switch (sh.nt * 8 + sh.cpl) {
+    case 256 * 8 + 1: return launch_oneshot_t<256, 1>(P, list, nblocks, lds_bytes, s);
+    case 256 * 8 + 2: return launch_oneshot_t<256, 2>(P, list, nblocks, lds_bytes, s);
+    case 256 * 8 + 4: return launch_oneshot_t<256, 4>(P, list, nblocks, lds_bytes, s);
+    case 512 * 8 + 1: return launch_oneshot_t<512, 1>(P, list, nblocks, lds_bytes, s);
+    case 512 * 8 + 2: return launch_oneshot_t<512, 2>(P, list, nblocks, lds_bytes, s);
+    case 512 * 8 + 4: return launch_oneshot_t<512, 4>(P, list, nblocks, lds_bytes, s);
     default: return hipErrorInvalidValue;
     }
 }

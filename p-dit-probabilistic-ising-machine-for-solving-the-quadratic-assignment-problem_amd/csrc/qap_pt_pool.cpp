@@ -448,8 +448,19 @@ void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throu
     // lazy updates hold u (|u| <= 2 max|D|) and v (|v| <= 4 max|F|) in 16 bits, split into
     // two signed bytes for the matrix cores (the high byte of 32640..32767 would not fit)
     const bool small_uv = 2 * bnd.maxD <= 32639 && 4 * bnd.maxF <= 32639;
-    if (pr.flags & QAP_PT_ONE_SHOT) {  // a location per thread, force matrix in HBM
-        h.shape = SweepShape{n <= 64 ? 64 : (n <= 128 ? 128 : 256), 1, IM_ONESHOT, sym};
+    if (pr.flags & QAP_PT_ONE_SHOT) {
+        // a location per thread, a candidate slot per lane (n <= 64 cpl), force matrix in
+        // HBM; eight waves (two per SIMD) hide the proposal loads and scans, 1.2-1.6x
+        // over four at n = 64 .. 256.  QAP_PT_SHAPE="nt,cpl" (nt 256 or 512) for tests
+        h.shape = SweepShape{512, n <= 64 ? 1 : (n <= 128 ? 2 : 4), IM_ONESHOT, sym};
+        if (const char *env = std::getenv("QAP_PT_SHAPE")) {
+            int nt = 0, cpl = 0;
+            if (std::sscanf(env, "%d,%d", &nt, &cpl) == 2 && sweep_shape_supported(nt, cpl, IM_ONESHOT) &&
+                nt >= n && 64 * cpl >= n) {
+                h.shape.nt = nt;
+                h.shape.cpl = cpl;
+            }
+        }
     } else {
         h.shape = choose_shape(n, sym, nonneg, bnd.force, sparse, small_uv, throughput, &ld_s);
     }

@@ -1009,7 +1009,7 @@ hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *l
 }
 
 bool sweep_shape_supported(int nt, int cpl, int im) {
-    if (im == IM_ONESHOT) return cpl == 1 && (nt == 64 || nt == 128 || nt == 256);
+    if (im == IM_ONESHOT) return (nt == 256 || nt == 512) && (cpl == 1 || cpl == 2 || cpl == 4);
     if (im == IM_LAZY && nt == 512 && (cpl == 1 || cpl == 2)) return true;
     switch (nt * 8 + cpl) {
     case 64 * 8 + 1: case 64 * 8 + 2: case 64 * 8 + 4: case 128 * 8 + 1: case 128 * 8 + 2:

@@ -138,6 +138,12 @@ CASES = {
     "os_grey_n256_R3": (lambda q: q.instances.grey_c2(), 3, 3, {"one_shot": True}),
     "os_n256_hot_R2": (lambda q: q.instances.mixed_instance(37, 256, True), 2, 2,
                        {"one_shot": True}),
+    "os_grey_256x4": (lambda q: q.instances.grey_c2(), 2, 3,
+                      {"one_shot": True, "env": {"QAP_PT_SHAPE": "256,4"}}),
+    "os_asym_n100_256x2": (lambda q: q.instances.random_instance(38, 100, 0, 30, symmetric=False),
+                           3, 4, {"one_shot": True, "env": {"QAP_PT_SHAPE": "256,2"}}),
+    "os_n50_256x1": (lambda q: q.instances.random_instance(39, 50), 3, 6,
+                     {"one_shot": True, "env": {"QAP_PT_SHAPE": "256,1"}}),
 }
 
 
