@@ -35,6 +35,8 @@ namespace qap {
 // [96] winner b, [97] winner delta, [98] winner's partner location j,
 // [140, 140 + 2 NW) per-wave rewritten-cell counts (uint64) at the end of a launch.
 constexpr int kRedWords = 160;
+// sparse flow updates with at most this many nonzero columns walk the columns directly
+constexpr int kWalkCols = 8;
 // p, pinv, cls, u1, u2, col, v1, v2, rnd, ddiag, drow x2, dcol x2, frow x2, ftrow x2,
 // vd1, vd2, fspan -- each round8(n) words
 constexpr int kSmallArrays = 21;
@@ -590,30 +592,15 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                         }
                     }
                     uint32_t nnz = (uint32_t)n;
-                    uint32_t base = 0;
-                    if (sparse_upd) {  // exact count of the nonzero columns
-                        const uint32_t inc = wave_incl_sum(nz);
-                        if constexpr (NW == 1) {
-                            base = inc - nz;
-                            nnz = (uint32_t)lane63((int)inc);
-                        } else {
-                            if (lane == kWave - 1) red_s[16 + wid] = (int)inc;
-                            lds_sync<NW>();
-                            uint32_t before = 0, tot = 0;
-#pragma unroll
-                            for (int ww = 0; ww < NW; ++ww) {
-                                const uint32_t x = (uint32_t)red_s[16 + ww];
-                                before += (ww < wid) ? x : 0u;
-                                tot += x;
-                            }
-                            base = before + inc - nz;
-                            nnz = tot;
-                        }
-                    }
                     const int c_lo = s_lo, c_hi = s_hi;
-                    const bool vec_path =
-                        IM != IM_HBM && (!sparse_upd || c_hi - c_lo + 1 <= 4 * (int)nnz + 4 * VW);
-                    if (vec_path) {  // v of every column for the vector path
+                    bool vec_path = IM != IM_HBM;  // dense flow: the range, 128-bit accesses
+                    // per-wave ballot masks of the nonzero columns, slot q of wave w at
+                    // word w * CPL + q (the pick's partial sums there are dead by now)
+                    uint64_t *msk_s = reinterpret_cast<uint64_t *>(red_s + 16);
+                    uint64_t mk[NW * CPL];  // the masks, read once (dependent LDS reads are slow)
+                    if (sparse_upd) {
+                        // v of every column and one mask per wave and slot, then one barrier:
+                        // every wave knows the exact nonzero count and the column set
 #pragma unroll
                         for (int q = 0; q < CPL; ++q) {
                             const int c = tid + q * NT;
@@ -621,19 +608,30 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                                 vd1_s[c] = v1[q];
                                 if constexpr (!SYM) vd2_s[c] = v2[q];
                             }
+                            const uint64_t bm = __ballot((v1[q] | v2[q]) != 0);
+                            if (lane == 0) msk_s[wid * CPL + q] = bm;
                         }
-                    } else if (sparse_upd) {  // compaction of the nonzero columns
+                        lds_sync<NW>();
+                        nnz = 0;
 #pragma unroll
-                        for (int q = 0; q < CPL; ++q) {
-                            if ((v1[q] | v2[q]) != 0) {
-                                col_s[base] = tid + q * NT;
-                                v1_s[base] = v1[q];
-                                v2_s[base] = v2[q];
-                                ++base;
+                        for (int m = 0; m < NW * CPL; ++m) {
+                            mk[m] = msk_s[m];
+                            nnz += (uint32_t)__popcll(mk[m]);
+                        }
+                        vec_path = IM != IM_HBM && c_hi - c_lo + 1 <= 4 * (int)nnz + 4 * VW;
+                    } else {
+                        if (vec_path) {  // v of every column for the vector path
+#pragma unroll
+                            for (int q = 0; q < CPL; ++q) {
+                                const int c = tid + q * NT;
+                                if (c < n4) {
+                                    vd1_s[c] = v1[q];
+                                    if constexpr (!SYM) vd2_s[c] = v2[q];
+                                }
                             }
                         }
+                        lds_sync<NW>();
                     }
-                    lds_sync<NW>();
                     {
                         uint32_t cnt = 0;
 #pragma unroll
@@ -659,7 +657,59 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                             dense_update<CPL, IM, SYM, true>(colp, rs, n, u1_s, u2_s, v1, v2);
                         else
                             dense_update<CPL, IM, SYM, false>(colp, rs, n, u1_s, u2_s, v1, v2);
-                    } else if (nnz > 0) {
+                    } else if (nnz <= (uint32_t)kWalkCols) {
+                        // few columns (a swap inside a block of equal flow rows changes two):
+                        // walk the set bits of the masks, all threads over the rows of each
+                        // The row factors are read once per row; a column costs one LDS round
+                        // trip (its v and the entries are independent reads).
+                        const int rs = (IM == IM_HBM) ? ld : ld_s;
+                        const bool m24 = d.mul24 != 0;
+                        for (int k0 = tid; k0 < n; k0 += NT) {
+                            const int32_t w1 = u1_s[k0];
+                            const int32_t w2 = SYM ? 0 : u2_s[k0];
+                            const int off = __mul24(k0, rs);
+#pragma unroll
+                            for (int m = 0; m < NW * CPL; ++m) {
+                                uint64_t word = mk[m];
+                                const int cbase = (m / CPL) * kWave + (m % CPL) * NT;
+                                while (word) {
+                                    const int c = cbase + __builtin_ctzll(word);
+                                    word &= word - 1;
+                                    IT *cp = Iaddr(0, c);
+                                    const int32_t x = (int32_t)cp[off];
+                                    const int32_t a1 = vd1_s[c];
+                                    int32_t dv = m24 ? __mul24(w1, a1) : w1 * a1;
+                                    if constexpr (!SYM) {
+                                        const int32_t a2 = vd2_s[c];
+                                        dv += m24 ? __mul24(w2, a2) : w2 * a2;
+                                    }
+                                    if constexpr (IM == IM_LDS16) cp[off] = (uint16_t)(x - dv);
+                                    else cp[off] = x + dv;
+                                }
+                            }
+                        }
+                    } else {
+                        // compacted column list, positions from the masks (no scan)
+                        uint32_t before = 0;
+#pragma unroll
+                        for (int m = 0; m < NW * CPL; ++m)
+                            before += (m < wid * CPL) ? (uint32_t)__popcll(mk[m]) : 0u;
+                        const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+                        for (int q = 0; q < CPL; ++q) {
+                            uint64_t mine = 0;
+#pragma unroll
+                            for (int m = 0; m < NW * CPL; ++m)
+                                if (m == wid * CPL + q) mine = mk[m];
+                            if ((v1[q] | v2[q]) != 0) {
+                                const uint32_t at = before + (uint32_t)__popcll(mine & lt);
+                                col_s[at] = tid + q * NT;
+                                v1_s[at] = v1[q];
+                                v2_s[at] = v2[q];
+                            }
+                            before += (uint32_t)__popcll(mine);
+                        }
+                        lds_sync<NW>();
                         int cw = 1;
                         while (cw < (int)nnz && cw < NT) cw <<= 1;
                         const int shw = __builtin_ctz(cw);
