@@ -58,6 +58,15 @@ def _assert_same(res, st, ver, ref):
     assert res["attempts"] == ref["attempts"]
 
 
+def _sparse_asym(seed, n, density):
+    """Asymmetric flow with nonzero diagonal and few entries per row (most swaps change
+    a handful of flow columns: the short column walk with both update terms)."""
+    g = np.random.default_rng(seed)
+    F = np.where(g.random((n, n)) < density, g.integers(1, 40, (n, n)), 0).astype(np.int32)
+    D = g.integers(0, 30, (n, n)).astype(np.int32)
+    return F, D
+
+
 CASES = {
     # name: (instance builder, R, sweeps, extra kwargs)
     "sym_n16_R4": (lambda q: q.instances.random_instance(11, 16), 4, 40, {}),
@@ -119,6 +128,11 @@ CASES = {
                            {"env": {"QAP_PT_SHAPE": "512,1"}}),
     "lazy_n700_512x2": (lambda q: q.instances.random_instance(34, 700), 2, 1,
                         {"env": {"QAP_PT_IMODE": "3", "QAP_PT_SHAPE": "512,2"}}),
+    # sparse asymmetric flow: short column walk (u2, v2 terms) in LDS and in HBM
+    "sparse_asym_n60": (lambda q: _sparse_asym(40, 60, 0.04), 4, 10, {}),
+    "sparse_asym_n60_hbm": (lambda q: _sparse_asym(41, 60, 0.04), 4, 10,
+                            {"env": {"QAP_PT_IMODE": "2"}}),
+    "sparse_asym_n200_lds16": (lambda q: _sparse_asym(42, 200, 0.03), 3, 3, {}),
     # one-shot parallel-proposal steps (qap.cu:1037-1609): 64 / 128 / 256 threads
     "os_sym_n16_R4": (lambda q: q.instances.random_instance(11, 16), 4, 40, {"one_shot": True}),
     "os_c1_n64_R16": (lambda q: q.instances.flat_c1(), 16, 12, {"one_shot": True}),
