@@ -23,8 +23,6 @@
 //     or over the compacted nonzero columns when F is sparse.
 // exchange_kernel: best fold, deterministic exchange pass and hot/cold kick of every
 // ladder of an instance (CPU_PIM.cpp:293-357), staged through LDS.
-#include <type_traits>
-
 #include "qap_pt_device.h"
 #include "qap_pt_launch.h"
 #include "qap_pt_sweep_common.h"
@@ -36,7 +34,9 @@ namespace qap {
 // [140, 140 + 2 NW) per-wave rewritten-cell counts (uint64) at the end of a launch.
 constexpr int kRedWords = 160;
 // sparse flow updates with at most this many nonzero columns walk the columns directly
-constexpr int kWalkCols = 8;
+// (one LDS round trip per column, in sequence); longer sets batch rows over a compacted
+// list, which is faster from three columns on (n = 100: 12.3 vs 10.5 M attempts/s at 8)
+constexpr int kWalkCols = 2;
 // p, pinv, cls, u1, u2, col, v1, v2, rnd, ddiag, drow x2, dcol x2, frow x2, ftrow x2,
 // vd1, vd2, fspan -- each round8(n) words
 constexpr int kSmallArrays = 21;
@@ -716,50 +716,41 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                         const int rw = NT >> shw;
                         const int cx = tid & (cw - 1);
                         const int ry = tid >> shw;
-                        const int rs = (IM == IM_HBM) ? ld : ld_s;
-                        const bool m24 = d.mul24 != 0;
-                        // Rows go G at a time per lane, G sized to the rows a lane owns (a
-                        // swap inside a block of equal flow rows leaves two nonzero columns:
-                        // two rows per lane).  Row offsets (< 2^20) and, when the host
-                        // allows, the products use the full-rate 24-bit multiplier; lanes
-                        // past the compacted list work on the padding column.
-                        auto cols_update = [&](auto g_tag) {
-                            constexpr int G = decltype(g_tag)::value;
-                            for (int ci0 = 0; ci0 < (int)nnz; ci0 += cw) {
-                                const int ci = ci0 + cx;
-                                const bool live = ci < (int)nnz;
-                                const int c = live ? col_s[ci] : n;
-                                const int32_t a1 = live ? v1_s[ci] : 0;
-                                const int32_t a2 = live ? v2_s[ci] : 0;
-                                IT *cp = Iaddr(0, c);
-                                for (int k0 = ry; k0 < n; k0 += G * rw) {
-                                    int32_t x[G], w1[G], w2[G];
+                        const int64_t rs = (IM == IM_HBM) ? (int64_t)ld : (int64_t)ld_s;
+                        constexpr int G = 8;
+                        // Rows go G at a time per lane with no bounds test inside a group;
+                        // lanes past the compacted list work on the padding column.  (A
+                        // per-element choice of the 24-bit multiplier here turned into a
+                        // branch per row: n = 100 sparse flow went from 12.3 to 10.6 M/s.)
+                        for (int ci0 = 0; ci0 < (int)nnz; ci0 += cw) {
+                            const int ci = ci0 + cx;
+                            const bool live = ci < (int)nnz;
+                            const int c = live ? col_s[ci] : n;
+                            const int32_t a1 = live ? v1_s[ci] : 0;
+                            const int32_t a2 = live ? v2_s[ci] : 0;
+                            IT *cp = Iaddr(0, c);
+                            for (int k0 = ry; k0 < n; k0 += G * rw) {
+                                int32_t x[G], w1[G], w2[G];
 #pragma unroll
-                                    for (int r = 0; r < G; ++r) {
-                                        const int k = min(k0 + r * rw, n - 1);
-                                        x[r] = (int32_t)cp[__mul24(k, rs)];
-                                        w1[r] = u1_s[k];
-                                        if constexpr (!SYM) w2[r] = u2_s[k];
-                                    }
+                                for (int r = 0; r < G; ++r) {
+                                    const int k = min(k0 + r * rw, n - 1);
+                                    x[r] = (int32_t)cp[k * rs];
+                                    w1[r] = u1_s[k];
+                                    if constexpr (!SYM) w2[r] = u2_s[k];
+                                }
 #pragma unroll
-                                    for (int r = 0; r < G; ++r) {
-                                        const int k = k0 + r * rw;
-                                        int32_t dv = m24 ? __mul24(w1[r], a1) : w1[r] * a1;
-                                        if constexpr (!SYM) dv += m24 ? __mul24(w2[r], a2) : w2[r] * a2;
-                                        if (k < n) {
-                                            if constexpr (IM == IM_LDS16)
-                                                cp[__mul24(k, rs)] = (uint16_t)(x[r] - dv);
-                                            else
-                                                cp[__mul24(k, rs)] = x[r] + dv;
-                                        }
+                                for (int r = 0; r < G; ++r) {
+                                    const int k = k0 + r * rw;
+                                    const int32_t dv = SYM ? w1[r] * a1 : w1[r] * a1 + w2[r] * a2;
+                                    if (k < n) {
+                                        if constexpr (IM == IM_LDS16)
+                                            cp[k * rs] = (uint16_t)(x[r] - dv);
+                                        else
+                                            cp[k * rs] = x[r] + dv;
                                     }
                                 }
                             }
-                        };
-                        const int rpl = (n - 1) / rw + 1;  // rows per lane
-                        if (rpl <= 2) cols_update(std::integral_constant<int, 2>{});
-                        else if (rpl <= 4) cols_update(std::integral_constant<int, 4>{});
-                        else cols_update(std::integral_constant<int, 8>{});
+                        }
                     }
                     if constexpr (IM == IM_HBM) __syncthreads();
                     else lds_sync<NW>();
