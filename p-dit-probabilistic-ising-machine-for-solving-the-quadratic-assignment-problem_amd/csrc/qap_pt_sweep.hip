@@ -19,8 +19,11 @@
 //   * Facilities with identical flow rows and columns share a class id; a swap inside
 //     one class changes neither the cost nor I, so the rank-2 update is skipped.
 //   * The rank-2 update I[k][c] += u1[k] v1[c] + u2[k] v2[c] (CPU_PIM.cpp:263-269) runs
-//     over owned columns with flow differences held in registers, four rows in flight,
-//     or over the compacted nonzero columns when F is sparse.
+//     over the static range of nonzero flow columns with 128-bit accesses (LDS
+//     placements), over owned columns with the flow differences in registers (HBM), or,
+//     when F is sparse, over the exact nonzero columns: per-wave ballot masks published
+//     with one barrier, one or two columns walked straight from the masks, more
+//     compacted into a list.
 // exchange_kernel: best fold, deterministic exchange pass and hot/cold kick of every
 // ladder of an instance (CPU_PIM.cpp:293-357), staged through LDS.
 #include "qap_pt_device.h"
@@ -29,7 +32,8 @@
 
 namespace qap {
 
-// red_s (ints): [0,16) max partials, [16,80) sum partials [q*16 + wave],
+// red_s (ints): [0,16) max partials, [16,80) sum partials [q*16 + wave] (in the commit of
+// a sparse update, [16, 16 + 2 NW CPL) holds the uint64 nonzero-column masks instead),
 // [96] winner b, [97] winner delta, [98] winner's partner location j,
 // [140, 140 + 2 NW) per-wave rewritten-cell counts (uint64) at the end of a launch.
 constexpr int kRedWords = 160;
