@@ -248,13 +248,14 @@ def auto_select(avg_energy, R: int, lower_prop: float = 0.1,
 
 
 def auto_beta(F, D, R: int, T: int = 1, sweeps_per_exchange: int = 1, seed: int = 0,
-              lower_prop: float = 0.1, higher_prop: float = 0.9, per_replica_init=False):
+              lower_prop: float = 0.1, higher_prop: float = 0.9, per_replica_init=False,
+              one_shot=False):
     """qap_pt_auto_beta: the automatic ladder scan (CPU_PIM.cpp:366-476) as one batched
-    run.  Returns (beta0_ln, ratio, avg_energy[AUTO_STEPS]); the ladder is then
-    beta_ln[l] = beta0_ln / ratio**(R - 1 - l)."""
+    run (one_shot: the trials run one-shot steps).  Returns (beta0_ln, ratio,
+    avg_energy[AUTO_STEPS]); the ladder is then beta_ln[l] = beta0_ln / ratio**(R - 1 - l)."""
     inst = Instance(F=F, D=D, beta_ln=np.ones(R), sweeps=0, T=T,
                     sweeps_per_exchange=sweeps_per_exchange, seed=seed,
-                    per_replica_init=per_replica_init)
+                    per_replica_init=per_replica_init, one_shot=one_shot)
     prob, par = inst.c_structs()
     avg = np.zeros(AUTO_STEPS, dtype=np.float64)
     b0, m = C.c_double(), C.c_double()

@@ -88,3 +88,30 @@ def test_cli_automatic_mode_runs(qpt, gpu, tmp_path):
     assert "Selecting minimum beta" in r.stdout and "Selecting maximum beta" in r.stdout
     best = int(re.search(r"Single best energy found: (-?\d+)", r.stdout).group(1))
     assert best >= 0
+
+
+def test_cli_one_shot_matches_oracle(qpt, oracle, gpu, tmp_path):
+    """--one-shot runs GPU-CUPIM's dynamics: the printed best is the oracle's one-shot run
+    for the same ladder, and the automatic scan's trials use the same dynamics."""
+    F, D = qpt.instances.random_instance(34, 18)
+    path = tmp_path / "r18.dat"
+    _write_dat(path, F, D)
+    R, T, w, i, seed = 5, 2, 60, 3, 99
+    cmd = [CLI, "--one-shot", "-b", "0.05", "-m", "1.4", "-r", str(R), "-t", str(T), "-w",
+           str(w), "-i", str(i), "--seed", str(seed), "-o", str(tmp_path / "o"), str(path)]
+    print("cli:", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    best = int(re.search(r"Single best energy found: (-?\d+)", r.stdout).group(1))
+    ref = oracle.solve(F, D, qpt.ladder_from(0.05, 1.4, R), w * i, T=T, spe=i, seed=seed,
+                       one_shot=True)
+    assert best == ref["best_cost"]
+    # automatic mode with one-shot trials: the scan's averages are the oracle's one-shot runs
+    b0, m, avg = qpt.auto_beta(F, D, 3, T=1, sweeps_per_exchange=1, seed=5, one_shot=True)
+    lad = np.full(3, 2.0 ** (-30 + 20) / 1.44269504089)
+    want = 0.0
+    for t in range(5):
+        rr = oracle.solve(F, D, lad, 10, T=1, spe=1, seed=_trial_seed(5, 20, t), one_shot=True)
+        for e in rr["final_E"]:
+            want += float(e) / (5.0 * 1 * 3)
+    assert avg[20] == want
