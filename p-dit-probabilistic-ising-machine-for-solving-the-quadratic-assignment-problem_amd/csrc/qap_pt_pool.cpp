@@ -97,14 +97,15 @@ bool is_sym_zero_diag(int n, const int32_t *M) {
 }
 
 int round4i(int x) { return (x + 3) & ~3; }
+}  // namespace
+
+namespace qap {
 
 // Row strides of the force matrix.  Every row has at least one padding column (index
 // n): lanes without a column of their own in the rank-2 update read and rewrite it, so
 // full row groups need no per-lane predication.  Both LDS copies have 16-byte rows (an
 // odd number of 4-word quads for int32, of 8-column octets for uint16) so the rank-2
 // update runs on 128-bit LDS accesses.
-}  // namespace
-namespace qap {
 int hbm_stride(int n) { return round4i(n + 1); }
 
 int device_cus() {
@@ -340,12 +341,6 @@ int validate(const qap_problem &pb, const qap_pt_params &pr, int k) {
     return QAP_PT_OK;
 }
 
-}  // namespace qap
-namespace {
-
-
-}  // namespace
-namespace qap {
 
 void pool_free(qap_pt_pool *p) {
     if (!p) return;
@@ -427,13 +422,14 @@ int build_graph(qap_pt_pool *p, int depth) {
     return QAP_PT_OK;
 }
 
+}  // namespace
+
+namespace qap {
+
 // Everything one instance needs on the device, built on the host: descriptor (offsets
 // left to the caller), kernel shape, matrices and their transposes, diagonals, facility
 // classes and flow spans, per-rung betas, and the initial permutations of every replica
 // (CPU_PIM.cpp:109-162).  Ladder positions start as the identity (order == pos).
-}  // namespace
-namespace qap {
-
 void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throughput,
                       HostInstance &h) {
     const int n = pb.n;

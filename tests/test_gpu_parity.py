@@ -67,6 +67,20 @@ def _sparse_asym(seed, n, density):
     return F, D
 
 
+def _wide(seed, n, wide_d, symmetric=True, density=0.15, big=4_300_000):
+    """One matrix 0/1 at the given density, the other with entries up to `big`, beyond
+    the 24-bit multiplier's range (2 max|D| or 4 max|F| >= 2^23): every update takes
+    the full 32-bit products.  The defaults at n = 96 start near a cost of 3e9, beyond
+    int32, with the force and delta bounds still inside the host's int32 check."""
+    g = np.random.default_rng(seed)
+    B = (g.random((n, n)) < density).astype(np.int32)
+    W = g.integers(0, big, (n, n)).astype(np.int32)
+    if symmetric:
+        B = np.triu(B, 1) + np.triu(B, 1).T
+        W = np.triu(W, 1) + np.triu(W, 1).T
+    return (B, W) if wide_d else (W, B)
+
+
 CASES = {
     # name: (instance builder, R, sweeps, extra kwargs)
     "sym_n16_R4": (lambda q: q.instances.random_instance(11, 16), 4, 40, {}),
@@ -133,6 +147,14 @@ CASES = {
     "sparse_asym_n60_hbm": (lambda q: _sparse_asym(41, 60, 0.04), 4, 10,
                             {"env": {"QAP_PT_IMODE": "2"}}),
     "sparse_asym_n200_lds16": (lambda q: _sparse_asym(42, 200, 0.03), 3, 3, {}),
+    # entries beyond the 24-bit multiplier, costs beyond int32 (beta scaled to the deltas)
+    "wide_d_n96": (lambda q: _wide(50, 96, True), 4, 6, {"beta_scale": 1e-5}),
+    "wide_d_asym_n96_hbm": (lambda q: _wide(51, 96, True, symmetric=False), 4, 6,
+                            {"beta_scale": 1e-5, "env": {"QAP_PT_IMODE": "2"}}),
+    "wide_f_n96_64x2": (lambda q: _wide(52, 96, False, density=0.2, big=2_500_000), 4, 6,
+                        {"beta_scale": 1e-5, "env": {"QAP_PT_SHAPE": "64,2"}}),
+    "wide_f_asym_n64": (lambda q: _wide(54, 64, False, symmetric=False, density=0.2,
+                                        big=2_500_000), 4, 8, {"beta_scale": 1e-5}),
     # one-shot parallel-proposal steps (qap.cu:1037-1609): 64 / 128 / 256 threads
     "os_sym_n16_R4": (lambda q: q.instances.random_instance(11, 16), 4, 40, {"one_shot": True}),
     "os_c1_n64_R16": (lambda q: q.instances.flat_c1(), 16, 12, {"one_shot": True}),
@@ -164,9 +186,11 @@ CASES = {
 @pytest.mark.parametrize("name", list(CASES))
 def test_trajectory_bit_exact(qpt, oracle, gpu, name):
     build, R, sweeps, kw = CASES[name]
+    kw = dict(kw)
     F, D = build(qpt)
     n = F.shape[0]
-    betas = qpt.instances.geometric_ladder(R, 1.0, 100.0) / max(1.0, n / 16.0)
+    betas = (qpt.instances.geometric_ladder(R, 1.0, 100.0) / max(1.0, n / 16.0)
+             * kw.pop("beta_scale", 1.0))
     print(f"case {name}: n={n} R={R} sweeps={sweeps} kw={kw}", flush=True)
     res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, sweeps, **kw)
     _assert_same(res, st, ver, ref)
