@@ -211,6 +211,12 @@ int main(int argc, char **argv) {
     qap_pt_result res{};
     res.best_perm = best_perm.data();
     int64_t shown = INT64_MAX;
+    // sweep<t>.txt for every ladder t (CPU_PIM.cpp:320,660-664): "megas elapsed_ms best"
+    // whenever the best over the ladder's replicas improved (replicas are ladder-major)
+    std::vector<std::ofstream> sweep_log(o.T);
+    for (int t = 0; t < o.T; ++t)
+        sweep_log[t].open(o.outdir + "/sweep" + std::to_string(t) + ".txt");
+    std::vector<int64_t> rep_best((size_t)o.R * o.T), ladder_shown(o.T, INT64_MAX);
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(o.megas, 100));
     for (int64_t m = 0; m < o.megas; m += chunk) {
         const int64_t step = std::min<int64_t>(chunk, o.megas - m);
@@ -221,8 +227,17 @@ int main(int argc, char **argv) {
             shown = res.best_cost;
             log << "Best Energy Found: " << shown << " Time (s): " << el << std::endl;
             std::cout << "Best Energy Found: " << shown << " Time (s): " << el << std::endl;
-            std::ofstream sw(o.outdir + "/sweep0.txt", std::ios::app);
-            sw << m + step << " " << (long long)(el * 1000) << " " << shown << std::endl;
+        }
+        if (check(qap_pt_pool_state(pool, 0, nullptr, nullptr, rep_best.data(), nullptr, nullptr,
+                                    nullptr)))
+            return 1;
+        for (int t = 0; t < o.T; ++t) {
+            const auto first = rep_best.begin() + (size_t)t * o.R;
+            const int64_t lb = *std::min_element(first, first + o.R);
+            if (lb < ladder_shown[t]) {
+                ladder_shown[t] = lb;
+                sweep_log[t] << m + step << " " << (long long)(el * 1000) << " " << lb << std::endl;
+            }
         }
         if (res.sweeps_done < (m + step) * o.numitera) break;  // goal or timeout
         if (o.timeout > 0 && el >= o.timeout) break;

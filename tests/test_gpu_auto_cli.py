@@ -73,7 +73,17 @@ def test_cli_best_cost_matches_oracle(qpt, oracle, gpu, tmp_path):
     assert qpt.instances.qap_cost(F, D, perm) == best
     log = (out / "log.txt").read_text()
     assert "Number of Replicas 6" in log and f"Single best energy found: {best}" in log
-    assert (out / "sweep0.txt").read_text().strip()
+    # one sweep<t>.txt per ladder (CPU_PIM.cpp:320,660-664): completed exchange intervals,
+    # milliseconds and the ladder's best, improving line by line; the last line is the
+    # best over that ladder's replicas at the end of the run
+    for t in range(T):
+        rows = [[int(v) for v in line.split()]
+                for line in (out / f"sweep{t}.txt").read_text().splitlines()]
+        assert rows, t
+        assert all(a[2] > b[2] and a[0] < b[0] for a, b in zip(rows, rows[1:])), rows
+        assert all(0 < r[0] <= w for r in rows)
+        assert rows[-1][2] == int(np.min(ref["final_bestE"][t * R:(t + 1) * R]))
+    assert not (out / f"sweep{T}.txt").exists()
 
 
 def test_cli_automatic_mode_runs(qpt, gpu, tmp_path):
