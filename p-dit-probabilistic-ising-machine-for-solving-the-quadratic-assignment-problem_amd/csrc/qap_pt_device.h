@@ -115,7 +115,9 @@ struct PoolDev {
     const int32_t *rep_inst;  // global replica -> instance
     const int32_t *F, *D, *Ft, *Dt;
     const int32_t *Fdiag, *Ddiag;
-    const int32_t *fclass;    // per facility: equal id <=> identical flow row and column
+    const int32_t *fclass;    // per facility: low 16 bits equal <=> identical flow row and
+                              //   column; high 16 bits equal <=> twins (symmetric flow rows
+                              //   equal outside the two facilities' own columns)
     const int32_t *fspan;     // per facility: nonzero columns of its flow row and column,
                               //   lo | hi << 10 | count << 20 (empty: lo 1023, hi 0, 0)
     const float *beta2f;      // per ladder position (exp2 units, float)

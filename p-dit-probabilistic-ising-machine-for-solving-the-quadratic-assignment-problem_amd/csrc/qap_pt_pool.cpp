@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/qap_pt.h"
+#include "qap_pt_flow.h"
 #include "qap_pt_launch.h"
 #include "qap_pt_internal.h"
 #include "qap_pt_pool_impl.h"
@@ -229,46 +230,6 @@ const char *placement_name(int im) {
     case IM_ONESHOT: return "oneshot";
     default: return "hbm";
     }
-}
-
-// Facility classes: equal id <=> identical flow column and row, i.e. swapping the two
-// facilities leaves every flow difference v1, v2 of the rank-2 update zero.
-std::vector<int32_t> facility_classes(int n, const int32_t *F) {
-    std::map<std::vector<int32_t>, int32_t> ids;
-    std::vector<int32_t> cls(n);
-    std::vector<int32_t> key(2 * (size_t)n);
-    for (int a = 0; a < n; ++a) {
-        for (int c = 0; c < n; ++c) {
-            key[c] = F[(int64_t)c * n + a];
-            key[n + c] = F[(int64_t)a * n + c];
-        }
-        auto it = ids.find(key);
-        if (it == ids.end()) it = ids.emplace(key, (int32_t)ids.size()).first;
-        cls[a] = it->second;
-    }
-    return cls;
-}
-
-// Nonzero columns of facility f's flow row and column, packed lo | hi << 10 | count << 20
-// (n <= 1024).  The flow differences v of a swap of a and b are zero outside
-// [min(lo_a, lo_b), max(hi_a, hi_b)] and have at most count_a + count_b nonzeros.
-int32_t facility_span(int n, const int32_t *F, int f) {
-    int lo = 1023, hi = 0, cnt = 0;
-    for (int c = 0; c < n; ++c)
-        if (F[(int64_t)f * n + c] != 0 || F[(int64_t)c * n + f] != 0) {
-            lo = std::min(lo, c);
-            hi = std::max(hi, c);
-            ++cnt;
-        }
-    return lo | (hi << 10) | (cnt << 20);
-}
-
-double offdiag_density(int n, const int32_t *F) {
-    if (n < 2) return 1.0;
-    int64_t nz = 0;
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) nz += (i != j) && F[(int64_t)i * n + j] != 0;
-    return (double)nz / ((double)n * (n - 1));
 }
 
 // Initial permutation (CPU_PIM.cpp:134-161): identity, then for i = 0..n-1 swap
@@ -494,7 +455,11 @@ void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throu
         h.Dd[i] = pb.D[(int64_t)i * n + i];
         h.span[i] = facility_span(n, pb.F, i);
     }
+    // class id (identical flow row and column) in the low 16 bits, twin class id (flow
+    // rows equal outside the two swapped columns; symmetric instances) in the high 16
     h.cls = facility_classes(n, pb.F);
+    const std::vector<int32_t> twin = facility_twins(n, pb.F, sym);
+    for (int i = 0; i < n; ++i) h.cls[i] |= twin[i] << 16;
     for (int l = 0; l < pr.R; ++l) {
         const double b2 = pr.beta_ln[l] * kLog2e;
         h.beta2f.push_back((float)b2);

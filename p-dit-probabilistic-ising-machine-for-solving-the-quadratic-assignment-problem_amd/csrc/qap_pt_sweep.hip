@@ -17,7 +17,11 @@
 //     registers one update ahead; the winning lane commits the permutation swap before
 //     the broadcast barrier, so a "stay" or a no-op swap costs three barriers.
 //   * Facilities with identical flow rows and columns share a class id; a swap inside
-//     one class changes neither the cost nor I, so the rank-2 update is skipped.
+//     one class changes neither the cost nor I, so the rank-2 update is skipped.  On
+//     symmetric instances, twins (flow rows equal outside their own two columns, e.g.
+//     two facilities of one dense block) change two columns of I: with I as uint16 in
+//     LDS or in HBM every thread updates its rows of those columns straight away, with
+//     no scan for nonzero columns.
 //   * The rank-2 update I[k][c] += u1[k] v1[c] + u2[k] v2[c] (CPU_PIM.cpp:263-269) runs
 //     over the static range of nonzero flow columns with 128-bit accesses (LDS
 //     placements), over owned columns with the flow differences in registers (HBM), or,
@@ -558,7 +562,31 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                 // (4) commit (CPU_PIM.cpp:252-283)
                 E -= dsel;
                 ++acc;
-                if (cls_s[a] != cls_s[b]) {
+                const int cla = cls_s[a], clb = cls_s[b];
+                // (not in the int32 LDS kernel: the extra code there cost 2.5-3.5 % on
+                // instances without twins, n = 64 dense and n = 100 sparse flow)
+                if (SYM && IM != IM_LDS32 && ((cla ^ clb) & 0xffff) != 0 && ((cla ^ clb) >> 16) == 0) {
+                    // twin facilities (symmetric flow rows equal outside columns a and b):
+                    // v = 2 (F[b] - F[a]) is 2 F[a][b] at column a, its negative at column
+                    // b and zero elsewhere, so the update is two columns of I += u v^T with
+                    // u = D[j] - D[i]; every thread updates its own rows, no column scan
+                    const int32_t f2 = 2 * frow[b];
+                    for (int k = tid; k < n; k += NT) {
+                        const int32_t dv = (D[(int64_t)j * n + k] - drow[k]) * f2;
+                        IT *ra = Iaddr(k, a), *rb = Iaddr(k, b);
+                        const int32_t xa = (int32_t)*ra, xb = (int32_t)*rb;
+                        if constexpr (IM == IM_LDS16) {  // -I stored
+                            *ra = (uint16_t)(xa - dv);
+                            *rb = (uint16_t)(xb + dv);
+                        } else {
+                            *ra = xa + dv;
+                            *rb = xb - dv;
+                        }
+                    }
+                    if (wid == 0) cells += 2 * (uint64_t)n;
+                    if constexpr (IM == IM_HBM) __syncthreads();
+                    else lds_sync<NW>();
+                } else if ((cla ^ clb) & 0xffff) {
                     // The flow differences are nonzero only inside the union of the two
                     // facilities' nonzero flow columns (static, from the host).  A dense
                     // instance updates that range with 128-bit accesses; a sparse one counts

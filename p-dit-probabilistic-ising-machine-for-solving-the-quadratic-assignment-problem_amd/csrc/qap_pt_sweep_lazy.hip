@@ -291,7 +291,7 @@ __global__ __launch_bounds__(NT) void sweep_lazy_kernel(PoolDev P, const int32_t
                 }
                 E -= dsel;  // (4) commit (CPU_PIM.cpp:252-283)
                 ++acc;
-                if (cls_s[a] != cls_s[b]) {
+                if (((cls_s[a] ^ cls_s[b]) & 0xffff) != 0) {  // flow class (low 16 bits)
                     // the move's u (rows) and v (columns) become pending update K
                     uint32_t cnt = 0;
 #pragma unroll
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(NT) void sweep_lazy_kernel(PoolDev P, const int32_t
                     dg_s[a] = dg_s[b];
                     dg_s[b] = t;
                 }
-                if (cls_s[a] == cls_s[b]) lds_sync<NW>();
+                if (((cls_s[a] ^ cls_s[b]) & 0xffff) == 0) lds_sync<NW>();
                 if (E < bestE) {  // CPU_PIM.cpp:271-272, plus the permutation
                     bestE = E;
                     for (int x = tid; x < n; x += NT) bestpg[x] = p_s[x];

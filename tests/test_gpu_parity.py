@@ -81,6 +81,31 @@ def _wide(seed, n, wide_d, symmetric=True, density=0.15, big=4_300_000):
     return (B, W) if wide_d else (W, B)
 
 
+def _twins(seed, n, blocks=((0, 8, 4), (8, 16, 9), (16, 24, 0)), pairs=((30, 31, 3), (33, 40, 6))):
+    """Symmetric zero-diagonal flow with planted twin sets (flow rows equal outside the
+    set's own columns): blocks [lo, hi) with pair value f and disjoint twin pairs (a, b, f).
+    A swap of twins changes two columns of the force matrix (one block with f = 0 has
+    identical rows: its swaps change nothing)."""
+    g = np.random.default_rng(seed)
+    F = np.triu(g.integers(0, 6, (n, n)), 1)
+    F = (F + F.T).astype(np.int32)
+    for lo, hi, f in blocks:
+        for a in range(lo + 1, hi):
+            for x in range(n):
+                if not lo <= x < hi:
+                    F[a, x] = F[x, a] = F[lo, x]
+        for a in range(lo, hi):
+            for b in range(lo, hi):
+                F[a, b] = f if a != b else 0
+    for a, b, f in pairs:
+        for x in range(n):
+            if x not in (a, b):
+                F[b, x] = F[x, b] = F[a, x]
+        F[a, b] = F[b, a] = f
+    D = np.triu(g.integers(0, 20, (n, n)), 1)
+    return F, (D + D.T).astype(np.int32)
+
+
 CASES = {
     # name: (instance builder, R, sweeps, extra kwargs)
     "sym_n16_R4": (lambda q: q.instances.random_instance(11, 16), 4, 40, {}),
@@ -147,6 +172,19 @@ CASES = {
     "sparse_asym_n60_hbm": (lambda q: _sparse_asym(41, 60, 0.04), 4, 10,
                             {"env": {"QAP_PT_IMODE": "2"}}),
     "sparse_asym_n200_lds16": (lambda q: _sparse_asym(42, 200, 0.03), 3, 3, {}),
+    # planted twin facilities: the two-column update (uint16 LDS and HBM placements; the
+    # int32 LDS kernel takes the general update on the same instances)
+    "twins_n48": (lambda q: _twins(60, 48), 4, 12, {}),
+    "twins_n48_hbm": (lambda q: _twins(61, 48), 4, 12, {"env": {"QAP_PT_IMODE": "2"}}),
+    "twins_n48_lds16": (lambda q: _twins(62, 48), 4, 12, {"env": {"QAP_PT_IMODE": "1"}}),
+    "twins_n48_64x1_T2_lds16": (lambda q: _twins(63, 48), 3, 12,
+                                {"T": 2, "env": {"QAP_PT_SHAPE": "64,1", "QAP_PT_IMODE": "1"}}),
+    "twins_n150_128x2_lds16": (lambda q: _twins(64, 150, blocks=((0, 40, 2), (40, 60, 0)),
+                                                pairs=((100, 149, 5),)), 3, 4,
+                               {"env": {"QAP_PT_SHAPE": "128,2", "QAP_PT_IMODE": "1"}}),
+    "twins_n150_256x1_hbm": (lambda q: _twins(65, 150, blocks=((10, 50, 3),),
+                                              pairs=((100, 101, 1), (7, 149, 2))), 3, 3,
+                             {"env": {"QAP_PT_SHAPE": "256,1", "QAP_PT_IMODE": "2"}}),
     # entries beyond the 24-bit multiplier, costs beyond int32 (beta scaled to the deltas)
     "wide_d_n96": (lambda q: _wide(50, 96, True), 4, 6, {"beta_scale": 1e-5}),
     "wide_d_asym_n96_hbm": (lambda q: _wide(51, 96, True, symmetric=False), 4, 6,
