@@ -140,8 +140,8 @@ namespace {
 // Threads x candidates per thread and force-matrix placement for size n.
 // Placement: QAP_PT_IMODE=0|1|2|3 when that one fits (LDS budget, uint16 range, lazy
 // conditions); else, for a throughput-bound pool, the fitting placement with the highest
-// tuned rate when the table rates all of them; else the first of lds32 -> lds16 -> lazy ->
-// hbm that fits.  Shape: QAP_PT_SHAPE="nt,cpl" (tests and
+// tuned rate among those the table rates; else the first of lds32 -> lds16 -> lazy -> hbm
+// that fits.  Shape: QAP_PT_SHAPE="nt,cpl" (tests and
 // the autotuner), else the tuned table's row for (n, flow class, placement), else a size
 // rule.
 SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool sparse,
@@ -163,28 +163,26 @@ SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool 
     else if (lazy) s.im = IM_LAZY;
     else s.im = IM_HBM;
     if (want < 0 && throughput) {
-        // A pool with more replicas than CUs is throughput-bound: with tuned rates for
-        // every placement that fits, take the fastest (a smaller LDS footprint lets more
-        // replicas share a CU).  A smaller pool is latency-bound and keeps the order above,
-        // which puts the force matrix in LDS whenever it fits.
+        // A pool with more replicas than CUs is throughput-bound: take the fastest rated
+        // placement that fits (a smaller LDS footprint lets more replicas share a CU).  A
+        // size bucket is rated at its largest n, so an LDS placement that fits only below
+        // that size has no rate there; it drops out (it needs the most LDS, the slowest
+        // at full load: n = 161..186 in an n <= 256 bank ran the int32 LDS kernel at a
+        // third of the lazy kernel's rate).  A smaller pool is latency-bound and keeps
+        // the order above, which puts the force matrix in LDS whenever it fits.
         const int cand[4] = {fit32 ? IM_LDS32 : -1, fit16 ? IM_LDS16 : -1, lazy ? IM_LAZY : -1,
                              IM_HBM};
         double best_rate = 0.0;
         int best_im = -1;
-        bool all_rated = true;
         for (int im : cand) {
             if (im < 0) continue;
             const TunedShape *t = tuned_shape(n, sparse, im);
-            if (!t || !(t->rate > 0.0)) {
-                all_rated = false;
-                break;
-            }
-            if (t->rate > best_rate) {
+            if (t && t->rate > best_rate) {
                 best_rate = t->rate;
                 best_im = im;
             }
         }
-        if (all_rated && best_im >= 0) s.im = best_im;
+        if (best_im >= 0) s.im = best_im;
     }
     *ld_s = lds_stride(n, s.im);
 
