@@ -46,8 +46,8 @@ constexpr int kRedWords = 160;
 // list, which is faster from three columns on (n = 100: 12.3 vs 10.5 M attempts/s at 8)
 constexpr int kWalkCols = 2;
 // p, pinv, cls, u1, u2, col, v1, v2, rnd, ddiag, drow x2, dcol x2, frow x2, ftrow x2,
-// vd1, vd2, fspan -- each round8(n) words
-constexpr int kSmallArrays = 21;
+// vd1, vd2, fspan, best permutation -- each round8(n) words
+constexpr int kSmallArrays = 22;
 
 __host__ __device__ inline int sweep_lds_words(int n, int ld_s, int im) {
     const int small = kSmallArrays * round8(n) + kRedWords;
@@ -249,7 +249,8 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     int32_t *vd1_s = ftrow_s + 2 * n4;                           // v of every column
     int32_t *vd2_s = vd1_s + n4;
     int32_t *fsp_s = vd2_s + n4;                                 // nonzero flow columns
-    int32_t *red_s = fsp_s + n4;
+    int32_t *bestp_s = fsp_s + n4;  // best permutation of this launch (written out at its end)
+    int32_t *red_s = bestp_s + n4;
     IT *I_s = reinterpret_cast<IT *>(red_s + kRedWords);
 
     const int32_t *__restrict__ F = P.F + d.mat_off;
@@ -339,6 +340,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
 
     int64_t E = P.E[g];
     int64_t bestE = P.bestE[g];
+    bool improved = false;  // bestp_s holds this launch's best permutation
     uint64_t acc = P.accepted[g];
     uint64_t cells = 0;  // this wave's share: n x the nonzero flow differences it owns
     const int pos = P.pos[g];
@@ -789,7 +791,14 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                 }
                 if (E < bestE) {  // CPU_PIM.cpp:271-272, plus the permutation
                     bestE = E;
-                    for (int x = tid; x < n; x += NT) bestpg[x] = p_s[x];
+                    improved = true;
+                    // uint16 placement: copied within LDS and written out at the end of
+                    // the launch (grey n = 256: 3.6 % faster than global stores in the
+                    // loop; in the int32 LDS kernel the LDS copy measured 0.5 % slower)
+                    if constexpr (IM == IM_LDS16)
+                        for (int x = tid; x < n; x += NT) bestp_s[x] = p_s[x];
+                    else
+                        for (int x = tid; x < n; x += NT) bestpg[x] = p_s[x];
                 }
             }
             // the prefetched rows become current
@@ -801,7 +810,10 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
 
     if (lane == 0) reinterpret_cast<uint64_t *>(red_s + 140)[wid] = cells;
     lds_sync<NW>();
-    for (int x = tid; x < n; x += NT) pg[x] = p_s[x];
+    for (int x = tid; x < n; x += NT) {
+        pg[x] = p_s[x];
+        if (IM == IM_LDS16 && improved) bestpg[x] = bestp_s[x];
+    }
 #pragma unroll
     for (int q = 0; q < CPL; ++q) {
         const int bb = tid + q * NT;
