@@ -108,3 +108,15 @@ def test_server_shutdown_fails_queued_requests(qpt):
     for _ in range(3):
         srv.submit(qpt.Instance(F=F, D=D, beta_ln=np.array([0.1, 0.2]), sweeps=2))
     srv.close()  # must return: queued requests are failed, running pools finish
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures the one-GPU path only: --gpus above 1 exits with a message
+    (status 2) before touching a device."""
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2, r.stderr
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
