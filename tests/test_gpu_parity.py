@@ -245,6 +245,33 @@ def test_goal_stops_like_oracle(qpt, oracle, gpu):
     _assert_same(res, st, ver, ref)
 
 
+def test_goal_stops_like_oracle_one_shot(qpt, oracle, gpu):
+    """A goal ends a one-shot run at the same exchange interval as the oracle's."""
+    F, D = qpt.instances.random_instance(41, 30)
+    betas = qpt.instances.geometric_ladder(6, 1.0, 50.0)
+    free = oracle.solve(F, D, betas, 60, seed=3, one_shot=True)
+    goal = int(free["best_cost"] * 1.02)
+    res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, 60, seed=3, goal=goal,
+                                  one_shot=True)
+    assert ref["megas_done"] < 60
+    assert res["sweeps_done"] == ref["megas_done"]
+    _assert_same(res, st, ver, ref)
+
+
+def test_incremental_runs_equal_one_run_one_shot(qpt, oracle, gpu):
+    F, D = qpt.instances.random_instance(42, 48)
+    betas = qpt.instances.geometric_ladder(8, 1.0, 100.0) / 3.0
+    inst = qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=30, seed=9, one_shot=True)
+    with qpt.Pool([inst]) as pool:
+        pool.run(10)
+        pool.run(20)
+        st = pool.state(0)
+    ref = oracle.solve(F, D, betas, 30, seed=9, one_shot=True)
+    np.testing.assert_array_equal(st["perm"], ref["final_perm"])
+    np.testing.assert_array_equal(st["E"], ref["final_E"])
+    np.testing.assert_array_equal(st["bestE"], ref["final_bestE"])
+
+
 def test_incremental_runs_equal_one_run(qpt, oracle, gpu):
     F, D = qpt.instances.random_instance(22, 48)
     betas = qpt.instances.geometric_ladder(8, 1.0, 100.0)
