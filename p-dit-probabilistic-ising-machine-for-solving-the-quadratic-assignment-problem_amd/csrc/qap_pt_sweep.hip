@@ -46,11 +46,12 @@ constexpr int kRedWords = 160;
 // list, which is faster from three columns on (n = 100: 12.3 vs 10.5 M attempts/s at 8)
 constexpr int kWalkCols = 2;
 // p, pinv, cls, u1, u2, col, v1, v2, rnd, ddiag, drow x2, dcol x2, frow x2, ftrow x2,
-// vd1, vd2, fspan, best permutation -- each round8(n) words
-constexpr int kSmallArrays = 22;
+// vd1, vd2, fspan -- each round8(n) words; the uint16 placement adds the launch's best
+// permutation
+constexpr int kSmallArrays = 21;
 
 __host__ __device__ inline int sweep_lds_words(int n, int ld_s, int im) {
-    const int small = kSmallArrays * round8(n) + kRedWords;
+    const int small = (kSmallArrays + (im == IM_LDS16 ? 1 : 0)) * round8(n) + kRedWords;
     if (im == IM_LDS32) return small + n * ld_s;
     if (im == IM_LDS16) return small + round4((n * ld_s + 1) / 2);
     return small;
@@ -249,8 +250,8 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     int32_t *vd1_s = ftrow_s + 2 * n4;                           // v of every column
     int32_t *vd2_s = vd1_s + n4;
     int32_t *fsp_s = vd2_s + n4;                                 // nonzero flow columns
-    int32_t *bestp_s = fsp_s + n4;  // best permutation of this launch (written out at its end)
-    int32_t *red_s = bestp_s + n4;
+    int32_t *bestp_s = fsp_s + n4;  // uint16 placement: this launch's best permutation
+    int32_t *red_s = bestp_s + (IM == IM_LDS16 ? n4 : 0);
     IT *I_s = reinterpret_cast<IT *>(red_s + kRedWords);
 
     const int32_t *__restrict__ F = P.F + d.mat_off;
