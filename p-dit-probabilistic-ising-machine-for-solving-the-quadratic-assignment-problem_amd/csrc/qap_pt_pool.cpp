@@ -124,6 +124,42 @@ int device_cus() {
 
 bool throughput_regime(int64_t replicas) { return replicas > device_cus(); }
 
+// Threads x candidates per thread for placement s.im at size n: the tuned table's row for
+// (n, flow class, placement), else a size rule; QAP_PT_SHAPE="nt,cpl" overrides both.
+void shape_for_size(int n, bool sparse, SweepShape &s) {
+    if (n <= 64) {
+        s.nt = 64;
+        s.cpl = 1;
+    } else if (n <= 128) {
+        s.nt = 128;
+        s.cpl = 1;
+    } else if (n <= 256) {
+        s.nt = 256;
+        s.cpl = 1;
+    } else if (n <= 512) {
+        s.nt = 256;
+        s.cpl = 2;
+    } else {
+        s.nt = 256;
+        s.cpl = 4;
+    }
+    if (s.im == IM_LAZY && n > 256) {  // eight waves keep more of the flush's loads in flight
+        s.nt = 512;
+        s.cpl = n <= 512 ? 1 : 2;
+    }
+    auto take = [&](int nt, int cpl) {
+        if (sweep_shape_supported(nt, cpl, s.im) && nt * cpl >= n) {
+            s.nt = nt;
+            s.cpl = cpl;
+        }
+    };
+    if (const TunedShape *t = tuned_shape(n, sparse, s.im)) take(t->nt, t->cpl);
+    if (const char *env = std::getenv("QAP_PT_SHAPE")) {
+        int nt = 0, cpl = 0;
+        if (std::sscanf(env, "%d,%d", &nt, &cpl) == 2) take(nt, cpl);
+    }
+}
+
 int lds_stride(int n, int im) {
     if (im == IM_LDS32) {  // 16-byte rows for 128-bit updates; odd number of 4-word quads
         int q = (n + 1 + 3) / 4;
@@ -185,38 +221,7 @@ SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool 
         if (best_im >= 0) s.im = best_im;
     }
     *ld_s = lds_stride(n, s.im);
-
-    if (n <= 64) {
-        s.nt = 64;
-        s.cpl = 1;
-    } else if (n <= 128) {
-        s.nt = 128;
-        s.cpl = 1;
-    } else if (n <= 256) {
-        s.nt = 256;
-        s.cpl = 1;
-    } else if (n <= 512) {
-        s.nt = 256;
-        s.cpl = 2;
-    } else {
-        s.nt = 256;
-        s.cpl = 4;
-    }
-    if (s.im == IM_LAZY && n > 256) {  // eight waves keep more of the flush's loads in flight
-        s.nt = 512;
-        s.cpl = n <= 512 ? 1 : 2;
-    }
-    auto take = [&](int nt, int cpl) {
-        if (sweep_shape_supported(nt, cpl, s.im) && nt * cpl >= n) {
-            s.nt = nt;
-            s.cpl = cpl;
-        }
-    };
-    if (const TunedShape *t = tuned_shape(n, sparse, s.im)) take(t->nt, t->cpl);
-    if (const char *env = std::getenv("QAP_PT_SHAPE")) {
-        int nt = 0, cpl = 0;
-        if (std::sscanf(env, "%d,%d", &nt, &cpl) == 2) take(nt, cpl);
-    }
+    shape_for_size(n, sparse, s);
     return s;
 }
 
