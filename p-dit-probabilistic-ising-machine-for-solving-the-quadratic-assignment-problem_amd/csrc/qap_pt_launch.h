@@ -33,6 +33,8 @@ bool sweep_shape_supported(int nt, int cpl, int im);
 int sweep_lds_bytes(int n, int ld_s, int im);
 // LDS of the lazy kernel for size n, and whether it holds at least 8 pending updates
 int lazy_lds_bytes(int n);
+// LDS of a lazy workgroup for size n holding at most max_pending (a multiple of 8) updates
+int lazy_lds_bytes(int n, int max_pending);
 bool lazy_fits(int n);
 // LDS of the one-shot kernel for size n, and whether one matrix pass holds 32 update vectors
 int oneshot_lds_bytes(int n);

@@ -22,6 +22,8 @@
 namespace qap {
 namespace {
 
+constexpr int kBankLazyPending = 24;  // pending rank-2 updates a lazy bank slot holds
+
 struct SlotLayout {
     int64_t mat, vec, beta, p, I, reps, ladders;
 };
@@ -264,7 +266,15 @@ int slots_launch(qap_pt_pool *p, int intervals, hipEvent_t done) {
                 b = it->second;
             }
             Bucket &B = p->buckets[b];
-            B.lds_bytes = std::max(B.lds_bytes, sweep_lds_bytes(d.n, d.ld_s, sh.im));
+            // lazy slots hold 24 pending updates instead of up to 32: the smaller LDS
+            // footprint lets more workgroups share a CU (n <= 256: three instead of two),
+            // which at a bank's load outweighs the extra flushes (served stream 245 -> 264 M
+            // attempts/s; a standalone 512-instance pool measured 12 % slower with 24 and
+            // keeps 32)
+            const int lds = sh.im == IM_LAZY ? std::min(sweep_lds_bytes(d.n, d.ld_s, sh.im),
+                                                        lazy_lds_bytes(d.n, kBankLazyPending))
+                                             : sweep_lds_bytes(d.n, d.ld_s, sh.im);
+            B.lds_bytes = std::max(B.lds_bytes, lds);
             for (int r = 0; r < d.R * d.T; ++r) B.reps.push_back(d.rep_base + r);
         }
         const int rt_cap = p->rt_cap;

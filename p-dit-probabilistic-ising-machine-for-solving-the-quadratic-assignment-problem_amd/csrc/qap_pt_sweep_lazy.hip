@@ -68,11 +68,15 @@ __device__ __forceinline__ int32_t dot8(v4h x, v4h y, int32_t acc) {
 
 }  // namespace
 
-int lazy_lds_bytes(int n) {
-    // kLazyMaxK entries in an odd number of 8-entry groups
-    const int want = 4 * lazy_fixed_words(n) + 4 * round8(n) * 8 * ((kLazyMaxK / 8) | 1);
+int lazy_lds_bytes(int n, int max_pending) {
+    // max_pending entries (a multiple of 8, at most kLazyMaxK) in an odd number of 8-entry
+    // groups
+    const int groups = (max_pending < kLazyMaxK ? max_pending : kLazyMaxK) / 8;
+    const int want = 4 * lazy_fixed_words(n) + 4 * round8(n) * 8 * (groups | 1);
     return want < kMaxLdsBytes ? want : kMaxLdsBytes;
 }
+
+int lazy_lds_bytes(int n) { return lazy_lds_bytes(n, kLazyMaxK); }
 
 bool lazy_fits(int n) { return lazy_layout(n, lazy_lds_bytes(n)).kmax >= 8; }
 
