@@ -46,7 +46,9 @@ hipError_t launch_sweep_lazy(const PoolDev &P, const SweepShape &sh, const int32
                              int nblocks, int lds_bytes, hipStream_t s);
 hipError_t launch_oneshot(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,
                           int lds_bytes, hipStream_t s);
-hipError_t launch_exchange(const PoolDev &P, int count, hipStream_t s);
+// max_rt: the largest replicas x ladders of an instance in the pool (staged in LDS up to
+// 2048)
+hipError_t launch_exchange(const PoolDev &P, int count, int max_rt, hipStream_t s);
 hipError_t launch_set_limits(InstState *state, const int64_t *limits, int count, hipStream_t s);
 hipError_t launch_halt(InstState *state, int count, hipStream_t s);
 hipError_t launch_force(const int32_t *F, const int32_t *D, const int32_t *perm, int n, int ld,

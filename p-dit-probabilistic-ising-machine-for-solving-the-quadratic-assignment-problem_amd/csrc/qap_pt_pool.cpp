@@ -357,7 +357,7 @@ int build_graph(qap_pt_pool *p, int depth) {
     HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int m = 0; m < depth && e == hipSuccess; ++m) {
-        e = launch_exchange(p->P, p->count, p->stream);
+        e = launch_exchange(p->P, p->count, p->max_rt, p->stream);
         if (e == hipSuccess && nb > 1) e = hipEventRecord(p->ev_fork, p->stream);
         for (int b = 1; b < nb && e == hipSuccess; ++b) {
             e = hipStreamWaitEvent(p->side[b - 1], p->ev_fork, 0);
@@ -591,6 +591,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
         p->total_reps += RT;
         p->total_ladders += d.T;
         p->total_beta += d.R;
+        p->max_rt = std::max(p->max_rt, RT);
         append(hF, h.F);
         append(hD, h.D);
         append(hFt, h.Ft);
@@ -770,7 +771,7 @@ int pool_run_impl(qap_pt_pool *p, int64_t sweeps, double *kernel_ms, int64_t *la
         while (m < max_new && rc == QAP_PT_OK) {
             int used = 0;
             for (int c = 0; c < chunk && m < max_new; ++c, ++m) {
-                if (launch_exchange(p->P, p->count, p->stream) != hipSuccess) {
+                if (launch_exchange(p->P, p->count, p->max_rt, p->stream) != hipSuccess) {
                     rc = fail(QAP_PT_E_DEVICE, "exchange launch failed");
                     break;
                 }

@@ -34,6 +34,7 @@ struct Bucket {
 struct qap_pt_pool {
     int device = 0;
     int count = 0;
+    int max_rt = 0;  // largest replicas x ladders of an instance (exchange staging)
     hipStream_t stream = nullptr;
     std::vector<qap_pt_params> params;
     std::vector<qap::InstDesc> desc;

@@ -68,6 +68,7 @@ int slots_create(int32_t slots, int32_t n_cap, int32_t r_cap, int32_t t_cap, int
     p->r_cap = r_cap;
     p->t_cap = t_cap;
     p->rt_cap = rt_cap;
+    p->max_rt = rt_cap;
     p->params.assign(slots, qap_pt_params{});
     p->desc.assign(slots, InstDesc{});
     p->shapes.assign(slots, SweepShape{});
@@ -287,7 +288,7 @@ int slots_launch(qap_pt_pool *p, int intervals, hipEvent_t done) {
         p->lists_dirty = false;
     }
     for (int m = 0; m < intervals; ++m) {
-        QAP_HIP_TRY(launch_exchange(p->P, p->count, p->stream));
+        QAP_HIP_TRY(launch_exchange(p->P, p->count, p->max_rt, p->stream));
         for (auto &B : p->buckets)
             if (!B.reps.empty())
                 QAP_HIP_TRY(launch_sweep(p->P, B.shape, B.d_list, (int)B.reps.size(), B.lds_bytes,

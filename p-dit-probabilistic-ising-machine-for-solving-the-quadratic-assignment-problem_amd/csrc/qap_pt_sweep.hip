@@ -879,13 +879,20 @@ __device__ __forceinline__ int64_t wave_min64(int64_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(64) void exchange_kernel(PoolDev P) {
-    __shared__ int64_t E_s[kExMaxRT];
-    __shared__ int64_t bE_s[kExMaxRT];
-    __shared__ int32_t ord_s[kExMaxRT];
-    __shared__ int32_t pos_s[kExMaxRT];
-    __shared__ int64_t gbest_s[64];
-    __shared__ uint32_t nex_s[64], nkick_s[64];
+// LDS of an exchange workgroup staging up to cap replicas: sized by the pool, so a launch
+// does not ask for room for kExMaxRT replicas beside the sweep workgroups of other pools
+// (with 50 KB a serving bank's exchange waited for CUs to free: 43 us per launch).
+__host__ __device__ inline int exchange_lds_bytes(int cap) { return 24 * cap + 1024; }
+
+__global__ __launch_bounds__(64) void exchange_kernel(PoolDev P, int cap) {
+    extern __shared__ __align__(16) unsigned char ex_smem[];
+    int64_t *E_s = reinterpret_cast<int64_t *>(ex_smem);
+    int64_t *bE_s = E_s + cap;
+    int64_t *gbest_s = bE_s + cap;  // [64]
+    int32_t *ord_s = reinterpret_cast<int32_t *>(gbest_s + 64);
+    int32_t *pos_s = ord_s + cap;
+    uint32_t *nex_s = reinterpret_cast<uint32_t *>(pos_s + cap);  // [64]
+    uint32_t *nkick_s = nex_s + 64;                                // [64]
     const int inst = blockIdx.x;
     const int tid = threadIdx.x;
     // Every load of the step is issued first, in one round (all of it is valid memory
@@ -893,7 +900,7 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P) {
     const InstDesc d = P.inst[inst];
     const InstState st = P.state[inst];
     const int R = d.R, T = d.T, RT = R * T;
-    const bool staged = RT <= kExMaxRT;
+    const bool staged = RT <= cap;
     int64_t *E = staged ? E_s : P.E + d.rep_base;
     int64_t *bE = staged ? bE_s : P.bestE + d.rep_base;
     int32_t *ord = staged ? ord_s : P.order + d.rep_base;
@@ -1109,8 +1116,9 @@ int sweep_lds_bytes(int n, int ld_s, int im) {
     return im == IM_LAZY ? lazy_lds_bytes(n) : 4 * sweep_lds_words(n, ld_s, im);
 }
 
-hipError_t launch_exchange(const PoolDev &P, int count, hipStream_t s) {
-    hipLaunchKernelGGL(exchange_kernel, dim3(count), dim3(64), 0, s, P);
+hipError_t launch_exchange(const PoolDev &P, int count, int max_rt, hipStream_t s) {
+    const int cap = max_rt < 1 ? 1 : (max_rt < kExMaxRT ? max_rt : kExMaxRT);
+    hipLaunchKernelGGL(exchange_kernel, dim3(count), dim3(64), exchange_lds_bytes(cap), s, P, cap);
     return hipGetLastError();
 }
 
