@@ -26,6 +26,9 @@ struct SweepShape {
     int cpl;     // candidates per thread: n <= nt * cpl
     int im;      // IMode
     bool sym;    // symmetric, zero-diagonal F and D
+    // every instance of the launch has sparse flow: the kernel built without the dense
+    // update path runs (set per bucket; any instance is still updated exactly there)
+    bool sparse = false;
 };
 
 // whether a kernel instance exists for nt threads x cpl candidates under placement im

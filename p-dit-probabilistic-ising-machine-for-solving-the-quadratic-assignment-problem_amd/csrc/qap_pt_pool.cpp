@@ -616,10 +616,12 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
             bucket_of[key] = bi;
             p->buckets.emplace_back();
             p->buckets[bi].shape = sh;
+            p->buckets[bi].shape.sparse = true;
         } else {
             bi = it->second;
         }
         Bucket &B = p->buckets[bi];
+        B.shape.sparse = B.shape.sparse && d.sparse_upd != 0;  // all-sparse buckets only
         B.lds_bytes = std::max(B.lds_bytes, sweep_lds_bytes(n, d.ld_s, sh.im));
         for (int r = 0; r < RT; ++r) B.reps.push_back(d.rep_base + r);
     }

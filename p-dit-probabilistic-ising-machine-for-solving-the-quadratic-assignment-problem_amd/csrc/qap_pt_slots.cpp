@@ -249,6 +249,7 @@ int slots_launch(qap_pt_pool *p, int intervals, hipEvent_t done) {
             key_of[std::make_tuple(sh.nt, sh.cpl, sh.im, (int)sh.sym)] = b;
             p->buckets[b].reps.clear();
             p->buckets[b].lds_bytes = 0;
+            p->buckets[b].shape.sparse = true;
         }
         for (int s = 0; s < p->count; ++s) {
             if (!p->used[s]) continue;
@@ -262,11 +263,13 @@ int slots_launch(qap_pt_pool *p, int intervals, hipEvent_t done) {
                 key_of[key] = b;
                 p->buckets.emplace_back();
                 p->buckets[b].shape = sh;
+                p->buckets[b].shape.sparse = true;
                 QAP_HIP_TRY(dalloc(&p->buckets[b].d_list, (size_t)p->total_reps));
             } else {
                 b = it->second;
             }
             Bucket &B = p->buckets[b];
+            B.shape.sparse = B.shape.sparse && d.sparse_upd != 0;  // all-sparse buckets only
             // lazy slots hold 24 pending updates instead of up to 32: the smaller LDS
             // footprint lets more workgroups share a CU (n <= 256: three instead of two),
             // which at a bank's load outweighs the extra flushes (served stream 245 -> 264 M

@@ -1,6 +1,6 @@
 // qap_pt_sweep.hip -- the hot path on gfx950: p-dit sweeps and replica exchange.
 //
-// sweep_kernel<NT, CPL, IM, SYM>: one workgroup of NT threads runs the sweeps of one
+// sweep_kernel<NT, CPL, IM, SYM, SP>: one workgroup of NT threads runs the sweeps of one
 // exchange interval for one replica (CPU_PIM.cpp:195-290; GPU_PIM.cu:304-453).
 //   * Candidate b of the heat-bath update belongs to thread b % NT (slot q = b / NT,
 //     CPL slots per thread), so row reads of the force matrix, flow row reads and the
@@ -219,7 +219,7 @@ struct IElem<IM_LDS16> {
     using T = uint16_t;  // holds -I
 };
 
-template <int NT, int CPL, int IM, bool SYM>
+template <int NT, int CPL, int IM, bool SYM, bool SP>
 __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__restrict__ rep_list) {
     constexpr int NW = NT / kWave;
     using IT = typename IElem<IM>::T;
@@ -348,7 +348,9 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     const float beta2f = P.beta2f[d.beta_off + pos];
     const double c20 = P.c20[d.beta_off + pos];
     const uint32_t sweep0 = P.state[inst].sweep_base;
-    const bool sparse_upd = d.sparse_upd != 0;
+    // SP: the launch holds sparse-flow instances only, and the dense update path is not
+    // built (grey n = 256: 2.2 % faster; the sparse path is exact for any flow)
+    const bool sparse_upd = SP || d.sparse_upd != 0;
 
     // F[b][b] of the owned candidates (constant for the launch) and their locations
     // pinv[b] (kept in registers, updated on every swap)
@@ -1056,14 +1058,14 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P, int cap) {
 // Launchers
 // ---------------------------------------------------------------------------
 template <int NT, int CPL, int IM, bool SYM>
-static hipError_t launch_sweep_t(const PoolDev &P, const int32_t *list, int nblocks, int lds_bytes,
-                                 hipStream_t s) {
-    auto k = sweep_kernel<NT, CPL, IM, SYM>;
-    static bool attr_done = false;
-    if (!attr_done) {
+static hipError_t launch_sweep_t(const PoolDev &P, bool sparse, const int32_t *list, int nblocks,
+                                 int lds_bytes, hipStream_t s) {
+    auto k = sparse ? sweep_kernel<NT, CPL, IM, SYM, true> : sweep_kernel<NT, CPL, IM, SYM, false>;
+    static bool attr_done[2] = {false, false};
+    if (!attr_done[sparse]) {
         (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   kMaxLdsBytes);
-        attr_done = true;
+        attr_done[sparse] = true;
     }
     hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds_bytes, s, P, list);
     return hipGetLastError();
@@ -1073,12 +1075,12 @@ template <int NT, int CPL>
 static hipError_t launch_sweep_c(const PoolDev &P, const SweepShape &sh, const int32_t *list,
                                  int nblocks, int lds_bytes, hipStream_t s) {
     switch (sh.im * 2 + (sh.sym ? 1 : 0)) {
-    case IM_LDS32 * 2 + 1: return launch_sweep_t<NT, CPL, IM_LDS32, true>(P, list, nblocks, lds_bytes, s);
-    case IM_LDS32 * 2 + 0: return launch_sweep_t<NT, CPL, IM_LDS32, false>(P, list, nblocks, lds_bytes, s);
-    case IM_LDS16 * 2 + 1: return launch_sweep_t<NT, CPL, IM_LDS16, true>(P, list, nblocks, lds_bytes, s);
-    case IM_LDS16 * 2 + 0: return launch_sweep_t<NT, CPL, IM_LDS16, false>(P, list, nblocks, lds_bytes, s);
-    case IM_HBM * 2 + 1: return launch_sweep_t<NT, CPL, IM_HBM, true>(P, list, nblocks, lds_bytes, s);
-    case IM_HBM * 2 + 0: return launch_sweep_t<NT, CPL, IM_HBM, false>(P, list, nblocks, lds_bytes, s);
+    case IM_LDS32 * 2 + 1: return launch_sweep_t<NT, CPL, IM_LDS32, true>(P, sh.sparse, list, nblocks, lds_bytes, s);
+    case IM_LDS32 * 2 + 0: return launch_sweep_t<NT, CPL, IM_LDS32, false>(P, sh.sparse, list, nblocks, lds_bytes, s);
+    case IM_LDS16 * 2 + 1: return launch_sweep_t<NT, CPL, IM_LDS16, true>(P, sh.sparse, list, nblocks, lds_bytes, s);
+    case IM_LDS16 * 2 + 0: return launch_sweep_t<NT, CPL, IM_LDS16, false>(P, sh.sparse, list, nblocks, lds_bytes, s);
+    case IM_HBM * 2 + 1: return launch_sweep_t<NT, CPL, IM_HBM, true>(P, sh.sparse, list, nblocks, lds_bytes, s);
+    case IM_HBM * 2 + 0: return launch_sweep_t<NT, CPL, IM_HBM, false>(P, sh.sparse, list, nblocks, lds_bytes, s);
     default: return hipErrorInvalidValue;
     }
 }
