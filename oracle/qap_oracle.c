@@ -5,7 +5,7 @@
  * library never links it.  The header lists the reference lines each part follows
  * and the pins that check it.  Plain C99 + POSIX threads, runtime sizes.
  *
- * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, so that float arithmetic is
+ * Build: oracle/Makefile (gcc -O3 -ffp-contract=off, so that float arithmetic is
  * exactly the IEEE sequence written here).
  */
 #include "qap_oracle.h"
@@ -93,16 +93,25 @@ double oracle_beta2(double beta_ln) { return beta_ln * 1.44269504089; }
 /* ------------------------------------------------------------------------- */
 void oracle_force_matrix(int32_t n, const int32_t *F, const int32_t *D, const int32_t *perm,
                          int32_t *I) {
+    /* I[i][a] = -sum_j (D[i][j] F[a][p(j)] + D[j][i] F[p(j)][a]) (CPU_PIM.cpp:164-181),
+     * accumulated in int64 one row i at a time: for each j, the flow column p(j) read as
+     * row p(j) of F^T and the flow row p(j), so every inner loop is unit-stride. */
+    int32_t *Ft = malloc(sizeof(int32_t) * (size_t)n * n);
+    int64_t *acc = malloc(sizeof(int64_t) * (size_t)n);
+    for (int x = 0; x < n; x++)
+        for (int y = 0; y < n; y++) Ft[(size_t)x * n + y] = F[(size_t)y * n + x];
     for (int i = 0; i < n; i++) {
-        for (int a = 0; a < n; a++) {
-            int64_t acc = 0;
-            for (int j = 0; j < n; j++) {
-                acc -= (int64_t)D[i * n + j] * F[a * n + perm[j]];
-                acc -= (int64_t)D[j * n + i] * F[perm[j] * n + a];
-            }
-            I[i * n + a] = (int32_t)acc;
+        for (int a = 0; a < n; a++) acc[a] = 0;
+        for (int j = 0; j < n; j++) {
+            const int64_t dij = D[(size_t)i * n + j], dji = D[(size_t)j * n + i];
+            const int32_t *fcol = Ft + (size_t)perm[j] * n; /* F[a][p(j)] over a */
+            const int32_t *frow = F + (size_t)perm[j] * n;  /* F[p(j)][a] over a */
+            for (int a = 0; a < n; a++) acc[a] -= dij * fcol[a] + dji * frow[a];
         }
+        for (int a = 0; a < n; a++) I[(size_t)i * n + a] = (int32_t)acc[a];
     }
+    free(acc);
+    free(Ft);
 }
 
 int64_t oracle_energy(int32_t n, const int32_t *F, const int32_t *D, const int32_t *perm) {
@@ -124,15 +133,20 @@ int64_t oracle_delta(int32_t n, const int32_t *F, const int32_t *D, const int32_
 
 void oracle_force_update(int32_t n, const int32_t *F, const int32_t *D, int32_t *I, int32_t i,
                          int32_t j, int32_t a, int32_t b) {
-    for (int k = 0; k < n; k++) {
-        int64_t u1 = (int64_t)D[k * n + j] - D[k * n + i];
-        int64_t u2 = (int64_t)D[j * n + k] - D[i * n + k];
-        for (int c = 0; c < n; c++) {
-            int64_t v1 = (int64_t)F[c * n + b] - F[c * n + a];
-            int64_t v2 = (int64_t)F[b * n + c] - F[a * n + c];
-            I[k * n + c] = (int32_t)(I[k * n + c] + u1 * v1 + u2 * v2);
-        }
+    /* I[k][c] += (D[k][j]-D[k][i]) (F[c][b]-F[c][a]) + (D[j][k]-D[i][k]) (F[b][c]-F[a][c])
+     * (CPU_PIM.cpp:263-269); the flow differences depend on c only and are formed once. */
+    int64_t *v = malloc(sizeof(int64_t) * 2 * (size_t)n);
+    for (int c = 0; c < n; c++) {
+        v[c] = (int64_t)F[(size_t)c * n + b] - F[(size_t)c * n + a];
+        v[n + c] = (int64_t)F[(size_t)b * n + c] - F[(size_t)a * n + c];
     }
+    for (int k = 0; k < n; k++) {
+        const int64_t u1 = (int64_t)D[(size_t)k * n + j] - D[(size_t)k * n + i];
+        const int64_t u2 = (int64_t)D[(size_t)j * n + k] - D[(size_t)i * n + k];
+        int32_t *row = I + (size_t)k * n;
+        for (int c = 0; c < n; c++) row[c] = (int32_t)(row[c] + u1 * v[c] + u2 * v[n + c]);
+    }
+    free(v);
 }
 
 void oracle_bounds(int32_t n, const int32_t *F, const int32_t *D, int64_t *force_bound,

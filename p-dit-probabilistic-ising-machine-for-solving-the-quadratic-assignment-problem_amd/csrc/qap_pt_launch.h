@@ -4,12 +4,33 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+#include <mutex>
+
 #include "qap_pt_device.h"
 
 namespace qap {
 
 // LDS available to one workgroup on gfx950 (160 KiB per CU).
 constexpr int kMaxLdsBytes = 160 * 1024;
+
+// Lets kernel K take up to kMaxLdsBytes of dynamic LDS.  Set once per process and kernel
+// instantiation; callers on several threads (a pool's owner, the serving worker) are
+// serialised by a per-kernel mutex, and a failure is returned, never remembered as done,
+// so the next launch tries again.
+template <auto K>
+hipError_t allow_full_lds() {
+    static std::atomic<bool> done{false};
+    static std::mutex mu;
+    if (done.load(std::memory_order_acquire)) return hipSuccess;
+    std::lock_guard<std::mutex> lock(mu);
+    if (done.load(std::memory_order_relaxed)) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(K),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             kMaxLdsBytes);
+    if (e == hipSuccess) done.store(true, std::memory_order_release);
+    return e;
+}
 
 // Where a replica's force matrix lives during a sweep launch.
 enum IMode : int {

@@ -1061,12 +1061,9 @@ template <int NT, int CPL, int IM, bool SYM>
 static hipError_t launch_sweep_t(const PoolDev &P, bool sparse, const int32_t *list, int nblocks,
                                  int lds_bytes, hipStream_t s) {
     auto k = sparse ? sweep_kernel<NT, CPL, IM, SYM, true> : sweep_kernel<NT, CPL, IM, SYM, false>;
-    static bool attr_done[2] = {false, false};
-    if (!attr_done[sparse]) {
-        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kMaxLdsBytes);
-        attr_done[sparse] = true;
-    }
+    const hipError_t e = sparse ? allow_full_lds<sweep_kernel<NT, CPL, IM, SYM, true>>()
+                                : allow_full_lds<sweep_kernel<NT, CPL, IM, SYM, false>>();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds_bytes, s, P, list);
     return hipGetLastError();
 }

@@ -382,12 +382,8 @@ template <int NT, int CPL>
 hipError_t launch_lazy_t(const PoolDev &P, const int32_t *list, int nblocks, int lds_bytes,
                          hipStream_t s) {
     auto k = sweep_lazy_kernel<NT, CPL>;
-    static bool attr_done = false;
-    if (!attr_done) {
-        (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kMaxLdsBytes);
-        attr_done = true;
-    }
+    const hipError_t e = allow_full_lds<sweep_lazy_kernel<NT, CPL>>();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds_bytes, s, P, list, lds_bytes);
     return hipGetLastError();
 }

@@ -218,6 +218,17 @@ CASES = {
                            3, 4, {"one_shot": True, "env": {"QAP_PT_SHAPE": "256,2"}}),
     "os_n50_256x1": (lambda q: q.instances.random_instance(39, 50), 3, 6,
                      {"one_shot": True, "env": {"QAP_PT_SHAPE": "256,1"}}),
+    # the benchmark configurations at their own sizes, shapes and ladders
+    # n = 1024 large dense: lazy placement, eight waves x 2 candidates, 24 pending updates
+    "lazy_c5_n1024_R2": (lambda q: q.instances.large_c5(1024), 2, 1,
+                         {"expect_shape": "512x2,lazy,sym"}),
+    # n = 64 flat: 16 replicas on T 1..100, a thousand exchange intervals
+    "c1_n64_R16_1000sweeps": (lambda q: q.instances.flat_c1(), 16, 1000,
+                              {"ladder": (1.0, 100.0), "expect_shape": "64x1,lds32,sym"}),
+    # grey pattern n = 256: all 64 replicas on T 10..1000 (the all-sparse uint16 build)
+    "grey_c2_n256_R64": (lambda q: q.instances.grey_c2(), 64, 3,
+                         {"ladder": (10.0, 1000.0),
+                          "expect_shape": "256x1,lds16,sym,sparse-flow"}),
 }
 
 
@@ -227,9 +238,17 @@ def test_trajectory_bit_exact(qpt, oracle, gpu, name):
     kw = dict(kw)
     F, D = build(qpt)
     n = F.shape[0]
-    betas = (qpt.instances.geometric_ladder(R, 1.0, 100.0) / max(1.0, n / 16.0)
-             * kw.pop("beta_scale", 1.0))
+    ladder = kw.pop("ladder", None)
+    expect_shape = kw.pop("expect_shape", None)
+    if ladder is None:
+        betas = (qpt.instances.geometric_ladder(R, 1.0, 100.0) / max(1.0, n / 16.0)
+                 * kw.pop("beta_scale", 1.0))
+    else:  # the workload's own ladder (temperatures t_min .. t_max)
+        betas = qpt.instances.geometric_ladder(R, *ladder)
     print(f"case {name}: n={n} R={R} sweeps={sweeps} kw={kw}", flush=True)
+    if expect_shape is not None:
+        with qpt.Pool([qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=0)]) as pool:
+            assert pool.shape(0).startswith(expect_shape), pool.shape(0)
     res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, sweeps, **kw)
     _assert_same(res, st, ver, ref)
 
@@ -349,3 +368,28 @@ def test_large_asym_n200_lds_shapes(qpt, oracle, gpu):
         res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, 2, seed=6,
                                       env={"QAP_PT_SHAPE": shape})
         _assert_same(res, st, ver, ref)
+
+
+def test_throughput_pool_more_replicas_than_cus(qpt, oracle, gpu):
+    """Twelve instances of the 512-instance batch (n 30..150, dense and sparse flow, 32
+    replicas each on T 100..10000): 384 replicas, more than the device's CUs, so the
+    placements come from the tuned full-chip rates and several kernel buckets share each
+    exchange interval.  Every instance's replica state equals its own oracle run."""
+    insts = []
+    for k, (F, D) in enumerate(qpt.instances.batch_c3()[:12]):
+        insts.append(qpt.Instance(F=F, D=D, beta_ln=qpt.instances.geometric_ladder(32, 100.0, 10000.0),
+                                  sweeps=6, seed=7000 + k))
+    assert sum(x.R for x in insts) > 256  # MI355X: 256 CUs
+    with qpt.Pool(insts) as pool:
+        shapes = [pool.shape(k) for k in range(len(insts))]
+        print("shapes:", shapes, flush=True)
+        # every one of these sizes fits the int32 LDS placement, which a latency-bound
+        # pool would take: a lazy placement shows the full-chip choice was made
+        assert any(",lazy," in s for s in shapes), shapes
+        pool.run(-1)
+        got = [(pool.result(k), pool.state(k)) for k in range(len(insts))]
+        fm, cm = pool.verify()
+    for k, inst in enumerate(insts):
+        ref = oracle.solve(inst.F, inst.D, inst.beta_ln, inst.sweeps, seed=inst.seed)
+        print(f"instance {k}: n={inst.n} shape={shapes[k]}", flush=True)
+        _assert_same(got[k][0], got[k][1], (fm, cm), ref)
