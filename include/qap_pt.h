@@ -103,6 +103,23 @@ int qap_pt_device_count(void);
 /* --- one-shot solve: replaces CPU_PIM/GPU_PIM main()'s solve path ------------- */
 int qap_pt_solve_hip(const qap_problem *prob, const qap_pt_params *par, qap_pt_result *res);
 
+/* --- allocation-free solve.  No counterpart in the reference: it is for callers that
+ *     cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc).
+ *     The same solve as qap_pt_solve_hip -- same arguments, results (bit-identical) and
+ *     errors -- but every device buffer of the solve is a piece of one fixed region of
+ *     QAP_PT_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from its first call on
+ *     that GPU until the process ends.  It calls no hipMalloc/hipFree, creates no stream,
+ *     event or graph: the kernels are launched one by one on `stream` (a hipStream_t;
+ *     NULL = the null stream) and the call returns once the results are copied out.  A
+ *     request whose device state exceeds the region fails with QAP_PT_E_NOMEM before any
+ *     device work (the message gives the bytes it needs).  Calls on one GPU share its
+ *     region, so two at once would race: the library serialises them with a per-device
+ *     lock.  The region keeps the last call's contents; each call writes everything it
+ *     reads first. ------------------------------------------------------------------- */
+#define QAP_PT_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int qap_pt_solve_hip_noalloc(const qap_problem *prob, const qap_pt_params *par,
+                             qap_pt_result *res, void *stream);
+
 /* --- batched solve: many independent instances (uneven n, own ladders) in the same
  *     device launches; results[k] belongs to problems[k]. ------------------------- */
 int qap_pt_solve_batch_hip(int32_t count, const qap_problem *problems,

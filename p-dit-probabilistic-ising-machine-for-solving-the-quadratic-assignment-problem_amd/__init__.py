@@ -28,6 +28,7 @@ QAP_PT_E_OVERFLOW = -4
 QAP_PT_E_BUSY = -5
 QAP_PT_INIT_PER_REPLICA = 1
 QAP_PT_ONE_SHOT = 2
+NOALLOC_SCRATCH_BYTES = 64 << 20  # QAP_PT_NOALLOC_SCRATCH_BYTES
 MAX_N = 1024
 AUTO_MIN_EXP = -30
 AUTO_STEPS = 32
@@ -37,6 +38,7 @@ EXPORTS = (
     "qap_pt_last_error",
     "qap_pt_device_count",
     "qap_pt_solve_hip",
+    "qap_pt_solve_hip_noalloc",
     "qap_pt_solve_batch_hip",
     "qap_pt_pool_create",
     "qap_pt_pool_run",
@@ -135,6 +137,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.qap_pt_device_count.restype = C.c_int
     lib.qap_pt_solve_hip.argtypes = [C.POINTER(QapProblem), C.POINTER(QapPtParams),
                                      C.POINTER(QapPtResult)]
+    lib.qap_pt_solve_hip_noalloc.argtypes = [C.POINTER(QapProblem), C.POINTER(QapPtParams),
+                                             C.POINTER(QapPtResult), C.c_void_p]
     lib.qap_pt_solve_batch_hip.argtypes = [C.c_int32, C.POINTER(QapProblem),
                                            C.POINTER(QapPtParams), C.POINTER(QapPtResult)]
     lib.qap_pt_pool_create.argtypes = [C.c_int32, C.POINTER(QapProblem),
@@ -292,6 +296,20 @@ def solve(inst: Instance) -> dict:
     res = QapPtResult()
     res.best_perm = _ptr(perm, C.c_int32)
     _check(lib.qap_pt_solve_hip(C.byref(prob), C.byref(par), C.byref(res)))
+    return _result_dict(res, perm)
+
+
+def solve_noalloc(inst: Instance, stream: int | None = None) -> dict:
+    """qap_pt_solve_hip_noalloc: the solve of `solve`, with every device buffer in the
+    library's fixed per-GPU region (no device allocation); kernels on `stream` (a
+    hipStream_t as an integer, None = the null stream)."""
+    lib = load_library()
+    prob, par = inst.c_structs()
+    perm = np.zeros(inst.n, dtype=np.int32)
+    res = QapPtResult()
+    res.best_perm = _ptr(perm, C.c_int32)
+    _check(lib.qap_pt_solve_hip_noalloc(C.byref(prob), C.byref(par), C.byref(res),
+                                        C.c_void_p(stream)))
     return _result_dict(res, perm)
 
 

@@ -312,6 +312,10 @@ void pool_free(qap_pt_pool *p) {
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->arena) {  // nothing here was allocated by the pool
+        delete p;
+        return;
+    }
     for (auto &b : p->buckets)
         if (b.d_list) (void)hipFree(b.d_list);
     void *bufs[] = {p->d_inst,  p->d_state,     p->d_rep_inst, p->d_F,      p->d_D, p->d_fclass, p->d_fspan,
@@ -324,7 +328,7 @@ void pool_free(qap_pt_pool *p) {
     for (auto st : p->side) (void)hipStreamDestroy(st);
     for (auto ev : p->ev_join) (void)hipEventDestroy(ev);
     if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
-    if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (p->stream && p->own_stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
 
@@ -541,8 +545,14 @@ void bind_pool(qap_pt_pool *p) {
 namespace {
 
 
+struct ArenaSpec {
+    char *base;
+    size_t size;
+    hipStream_t stream;
+};
+
 int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_params *params,
-                     qap_pt_pool **out) {
+                     qap_pt_pool **out, const ArenaSpec *arena = nullptr) {
     if (!out) return fail(QAP_PT_E_INVAL, "out is NULL");
     *out = nullptr;
     if (count < 1 || !problems || !params) return fail(QAP_PT_E_INVAL, "empty instance list");
@@ -556,6 +566,13 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
 
     qap_pt_pool *p = new qap_pt_pool();
     (void)hipGetDevice(&p->device);
+    if (arena) {
+        p->arena = arena->base;
+        p->arena_size = arena->size;
+        p->own_stream = false;
+        p->plain_launch = true;
+        p->stream = arena->stream;
+    }
     p->count = count;
     p->params.assign(params, params + count);
     p->desc.resize(count);
@@ -640,39 +657,43 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
                              "%s failed: %s", #expr, hipGetErrorString(e_)));                    \
     } while (0)
 
-    POOL_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    if (p->own_stream) POOL_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
     const int RTall = p->total_reps;
-    POOL_TRY(dalloc(&p->d_inst, count));
-    POOL_TRY(dalloc(&p->d_state, count));
-    POOL_TRY(dalloc(&p->d_limits, count));
-    POOL_TRY(dalloc(&p->d_rep_inst, RTall));
-    POOL_TRY(dalloc(&p->d_F, p->total_mat));
-    POOL_TRY(dalloc(&p->d_D, p->total_mat));
-    POOL_TRY(dalloc(&p->d_Ft, p->total_mat));
-    POOL_TRY(dalloc(&p->d_Dt, p->total_mat));
-    POOL_TRY(dalloc(&p->d_Fdiag, p->total_vec));
-    POOL_TRY(dalloc(&p->d_Ddiag, p->total_vec));
-    POOL_TRY(dalloc(&p->d_fclass, p->total_vec));
-    POOL_TRY(dalloc(&p->d_fspan, p->total_vec));
-    POOL_TRY(dalloc(&p->d_beta2f, p->total_beta));
-    POOL_TRY(dalloc(&p->d_c20, p->total_beta));
-    POOL_TRY(dalloc(&p->d_perm, p->total_p));
-    POOL_TRY(dalloc(&p->d_pinv, p->total_p));
-    POOL_TRY(dalloc(&p->d_best_perm, p->total_p));
-    POOL_TRY(dalloc(&p->d_I, p->total_I));
-    POOL_TRY(dalloc(&p->d_E, RTall));
-    POOL_TRY(dalloc(&p->d_bestE, RTall));
-    POOL_TRY(dalloc(&p->d_accepted, RTall));
-    POOL_TRY(dalloc(&p->d_cells, RTall));
-    POOL_TRY(dalloc(&p->d_pos, RTall));
-    POOL_TRY(dalloc(&p->d_order, RTall));
-    POOL_TRY(dalloc(&p->d_tss, p->total_ladders));
-    POOL_TRY(dalloc(&p->d_ladder_best, p->total_ladders));
-    for (auto &b : p->buckets) {
-        POOL_TRY(dalloc(&b.d_list, b.reps.size()));
+    POOL_TRY(palloc(p, &p->d_inst, count));
+    POOL_TRY(palloc(p, &p->d_state, count));
+    POOL_TRY(palloc(p, &p->d_limits, count));
+    POOL_TRY(palloc(p, &p->d_rep_inst, RTall));
+    POOL_TRY(palloc(p, &p->d_F, p->total_mat));
+    POOL_TRY(palloc(p, &p->d_D, p->total_mat));
+    POOL_TRY(palloc(p, &p->d_Ft, p->total_mat));
+    POOL_TRY(palloc(p, &p->d_Dt, p->total_mat));
+    POOL_TRY(palloc(p, &p->d_Fdiag, p->total_vec));
+    POOL_TRY(palloc(p, &p->d_Ddiag, p->total_vec));
+    POOL_TRY(palloc(p, &p->d_fclass, p->total_vec));
+    POOL_TRY(palloc(p, &p->d_fspan, p->total_vec));
+    POOL_TRY(palloc(p, &p->d_beta2f, p->total_beta));
+    POOL_TRY(palloc(p, &p->d_c20, p->total_beta));
+    POOL_TRY(palloc(p, &p->d_perm, p->total_p));
+    POOL_TRY(palloc(p, &p->d_pinv, p->total_p));
+    POOL_TRY(palloc(p, &p->d_best_perm, p->total_p));
+    POOL_TRY(palloc(p, &p->d_I, p->total_I));
+    POOL_TRY(palloc(p, &p->d_E, RTall));
+    POOL_TRY(palloc(p, &p->d_bestE, RTall));
+    POOL_TRY(palloc(p, &p->d_accepted, RTall));
+    POOL_TRY(palloc(p, &p->d_cells, RTall));
+    POOL_TRY(palloc(p, &p->d_pos, RTall));
+    POOL_TRY(palloc(p, &p->d_order, RTall));
+    POOL_TRY(palloc(p, &p->d_tss, p->total_ladders));
+    POOL_TRY(palloc(p, &p->d_ladder_best, p->total_ladders));
+    for (auto &b : p->buckets) POOL_TRY(palloc(p, &b.d_list, b.reps.size()));
+    if (p->arena && p->arena_used > p->arena_size)
+        return bail(fail(QAP_PT_E_NOMEM,
+                         "the instances need %zu bytes of device memory; the fixed scratch "
+                         "region holds %zu",
+                         p->arena_used, p->arena_size));
+    for (auto &b : p->buckets)
         POOL_TRY(hipMemcpy(b.d_list, b.reps.data(), b.reps.size() * sizeof(int32_t),
                            hipMemcpyHostToDevice));
-    }
 
     std::vector<InstState> st(count);
     for (auto &s : st) {
@@ -708,7 +729,8 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(up(p->d_order, order.data(), 4 * order.size()));
     POOL_TRY(up(p->d_tss, tss.data(), 4 * tss.size()));
     POOL_TRY(up(p->d_ladder_best, ladder_best.data(), 8 * ladder_best.size()));
-    POOL_TRY(hipMemset(p->d_I, 0, sizeof(int32_t) * std::max<int64_t>(p->total_I, 1)));
+    POOL_TRY(hipMemsetAsync(p->d_I, 0, sizeof(int32_t) * std::max<int64_t>(p->total_I, 1),
+                            p->stream));
 
     bind_pool(p);
 
@@ -810,12 +832,23 @@ int pool_run_impl(qap_pt_pool *p, int64_t sweeps, double *kernel_ms, int64_t *la
         if (launches) *launches = nl;
     } else {
         const int depth = (int)std::max<int64_t>(1, std::min<int64_t>(max_new, 32));
-        if (max_new > 0 && (!p->graph_exec || p->graph_depth != depth)) {
+        if (!p->plain_launch && max_new > 0 && (!p->graph_exec || p->graph_depth != depth)) {
             const int rc = build_graph(p, depth);
             if (rc) return rc;
         }
-        for (int64_t m = 0; m < max_new; m += p->graph_depth) {
-            HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
+        for (int64_t m = 0; m < max_new; m += depth) {
+            if (p->plain_launch) {
+                // the graph's launches issued one by one on the pool stream: exchange,
+                // then every bucket in turn (no side streams, events or graph objects)
+                for (int64_t x = m; x < std::min<int64_t>(max_new, m + depth); ++x) {
+                    HIP_TRY(launch_exchange(p->P, p->count, p->max_rt, p->stream));
+                    for (auto &b : p->buckets)
+                        HIP_TRY(launch_sweep(p->P, b.shape, b.d_list, (int)b.reps.size(),
+                                             b.lds_bytes, p->stream));
+                }
+            } else {
+                HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
+            }
             if (any_goal || timeout > 0) {
                 HIP_TRY(hipStreamSynchronize(p->stream));
                 if (timeout > 0 && elapsed() > timeout) {
@@ -933,6 +966,16 @@ int pool_result_internal(qap_pt_pool *p, int inst, qap_pt_result *res) {
 }
 
 void pool_destroy_internal(qap_pt_pool *p) { pool_free(p); }
+
+int pool_create_in_arena(int32_t count, const qap_problem *problems, const qap_pt_params *params,
+                         void *arena, size_t size, hipStream_t stream, qap_pt_pool **out) {
+    const ArenaSpec a{static_cast<char *>(arena), size, stream};
+    try {
+        return pool_create_impl(count, problems, params, out, &a);
+    } catch (const std::bad_alloc &) {
+        return fail(QAP_PT_E_NOMEM, "host allocation failed");
+    }
+}
 
 int64_t pool_device_bytes(int32_t count, const qap_problem *problems, const qap_pt_params *params) {
     int64_t bytes = 0;

@@ -72,6 +72,14 @@ struct qap_pt_pool {
     std::vector<qap::InstState> st_load;         // per slot: the state uploaded by its load
     std::vector<std::vector<int64_t>> fill64;  // per slot: INT64_MAX / zero upload sources
     bool lists_dirty = false;
+    // arena pool: every device buffer is a 256-byte-aligned piece of a region the library
+    // does not allocate (arena != nullptr); such a pool runs on the caller's stream
+    // (own_stream false), launches without graphs, events or side streams
+    // (plain_launch), and frees nothing
+    char *arena = nullptr;
+    size_t arena_size = 0, arena_used = 0;
+    bool own_stream = true;
+    bool plain_launch = false;
 };
 
 
@@ -92,6 +100,27 @@ hipError_t dalloc(T **p, size_t count) {
     if (count == 0) count = 1;
     return hipMalloc((void **)p, count * sizeof(T));
 }
+
+// Device buffer of `count` elements for pool p: hipMalloc, or for an arena pool the next
+// 256-byte-aligned piece of the arena.  An arena that is too small only advances
+// arena_used past arena_size (the caller checks once, after the last piece, and reports
+// the exact size needed) and hands out nullptr.
+template <typename T>
+hipError_t palloc(qap_pt_pool *p, T **ptr, size_t count) {
+    if (!p->arena) return dalloc(ptr, count);
+    *ptr = nullptr;
+    if (count == 0) count = 1;
+    const size_t off = (p->arena_used + 255) & ~(size_t)255;
+    p->arena_used = off + count * sizeof(T);
+    if (p->arena_used <= p->arena_size) *ptr = reinterpret_cast<T *>(p->arena + off);
+    return hipSuccess;
+}
+
+// A pool whose buffers come from `arena` (size bytes, device memory the caller keeps) and
+// whose work goes to `stream`; QAP_PT_E_NOMEM, before any device work, when the instances
+// need more than `size`.
+int pool_create_in_arena(int32_t count, const qap_problem *problems, const qap_pt_params *params,
+                         void *arena, size_t size, hipStream_t stream, qap_pt_pool **out);
 
 int validate(const qap_problem &pb, const qap_pt_params &pr, int k);
 // throughput: the instance runs in a pool with more replicas than the device has CUs (the
