@@ -1,15 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X parallel-tempering QAP annealer (one GPU).
 
-A step is one pass of the hot path over the workload: `--sweeps-per-step` sweeps of
-every replica (n p-dit updates = swap attempts each), with a replica exchange after
-every sweep.  W untimed steps, then K timed steps; the state (flow/distance,
-permutations, force matrices) is resident in HBM before the timed region starts.
+Default (`--mode batched`): the 512-instance batch (n uniform in [30,150], half dense and
+half sparse flow, Euclidean distance, 32 replicas each, one exchange per sweep).  A step
+is `--sweeps-per-step` sweeps of every replica of every instance (a sweep = n p-dit
+updates = n swap attempts per replica), run through the production path: one pool, its
+exchange intervals replayed as captured hipGraphs.  W untimed steps run on a separate
+pool of the same instances first; then K timed steps on a fresh pool whose instances are
+already resident in HBM.  Beside it:
+  * the plain path: one instance per pool (one instance per launch), run one after
+    another, on an evenly spaced sample of the instances, for the same K steps;
+  * time-to-target on that sample: the first step end at which an instance's best cost is
+    within 1 % of the best cost of the oracle's reference-arithmetic run (the CPU_PIM
+    floating-point roulette) at the same number of sweeps; the plain path's clock runs
+    from the start of the whole queue, so the instances before an instance count;
+  * the roofline of the dominant sweep kernel from a timed pass (HIP events on the stream
+    each launch goes to) after the timed steps;
+  * the CPU baseline: the oracle's reference-arithmetic mode on the host cores.
 
-Prints one JSON line: swap-attempts/s (whole job), the roofline of the sweep kernel
-(algorithmic bytes per launch / mean launch time from HIP events on the launching
-stream), and a CPU baseline (the oracle's reference-faithful floating-point mode on
-the host cores, a bounded sample of the same workload).
+`--mode throughput --workload grey256|flat64|large512|large1024|batch512` gives the one-
+workload throughput line; `--mode served` and `--mode dynamics` the serving-loop and
+one-shot-chain comparisons.  Prints one JSON line.
 """
 import argparse
 import importlib.util
@@ -22,7 +33,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(
     ROOT, "p-dit-probabilistic-ising-machine-for-solving-the-quadratic-assignment-problem_amd")
 
+METRIC = "swap-attempts/s; p50/p99 time-to-target (batched/served vs plain PT), 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+# LDS roof: 256 CUs x 128 bytes per clock x 2.4 GHz peak engine clock (MI355X)
+MI355X_CUS = 256
+MI355X_CLOCK_GHZ = 2.4
+LDS_PEAK_GBS = MI355X_CUS * 128 * MI355X_CLOCK_GHZ
 
 
 def load_pkg():
@@ -36,75 +52,22 @@ def load_pkg():
     return mod
 
 
-def workload(pkg, name):
-    """Returns (instances, description dict, cpu sample spec)."""
-    I = pkg.instances
-    if name == "grey256":
-        F, D = I.grey_c2()
-        betas = I.geometric_ladder(64, 10.0, 1000.0)
-        inst = [pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=0, seed=2024)]
-        desc = {"workload": "grey-pattern n=256 (92x92 unit-flow block, squared torus "
-                            "Manhattan distance 16x16), 64 replicas, geometric T 10..1000, "
-                            "exchange every sweep", "n": 256, "replicas": 64, "instances": 1}
-        return inst, desc, {"ladders_per_thread": 1, "sweeps": 4}
-    if name == "flat64":
-        F, D = I.flat_c1()
-        betas = I.geometric_ladder(16, 1.0, 100.0)
-        inst = [pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=0, seed=2024)]
-        desc = {"workload": "n=64 uniform flow [0,99], rounded Euclidean distance, 16 "
-                            "replicas, geometric T 1..100, exchange every sweep", "n": 64,
-                "replicas": 16, "instances": 1}
-        return inst, desc, {"ladders_per_thread": 1, "sweeps": 4000}
-    if name == "batch512":
-        insts = []
-        for k, (F, D) in enumerate(I.batch_c3()):
-            betas = I.geometric_ladder(32, 100.0, 10000.0)
-            insts.append(pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=0, seed=7000 + k))
-        desc = {"workload": "512 instances, n uniform [30,150], half dense / half sparse "
-                            "flow, Euclidean distance, 32 replicas each, geometric T "
-                            "100..10000, exchange every sweep, one launch per step",
-                "n": "30..150", "replicas": 32 * 512, "instances": 512}
-        return insts, desc, {"ladders_per_thread": 1, "sweeps": 4}
-    if name in ("large512", "large1024"):
-        n = 512 if name == "large512" else 1024
-        F, D = I.large_c5(n)
-        betas = I.geometric_ladder(64, 300.0, 30000.0)
-        inst = [pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=0, seed=2024)]
-        desc = {"workload": f"n={n} flow U[0,255], Manhattan grid distance, 64 replicas, "
-                            "geometric T 300..30000, exchange every sweep", "n": n,
-                "replicas": 64, "instances": 1}
-        return inst, desc, {"ladders_per_thread": 1, "sweeps": 1}
-    raise SystemExit(f"unknown workload {name}")
-
-
-def algorithmic_bytes(attempts, n_weighted_attempts, accepted_n, cells):
-    """24 n bytes per attempt (six int32 n-vectors: force row, gathered force column,
-    gathered force diagonal, distance row, flow row, inverse permutation) plus, per
-    accepted move, 8 bytes per rewritten force entry (read + write) and 16 n bytes for
-    the two difference vectors."""
-    return 24 * n_weighted_attempts + 8 * cells + 16 * accepted_n
-
-
-def cpu_baseline(pkg, insts, spec, threads):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-
-    inst = insts[0]
-    sweeps = spec["sweeps"]
-    t0 = time.perf_counter()
-    r = pyoracle.solve(inst.F, inst.D, inst.beta_ln, sweeps, T=threads, seed=inst.seed,
-                       mode=pyoracle.MODE_REFFP, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return {
-        "value": r["attempts"] / dt,
-        "unit": "swap-attempts/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"{threads} independent ladders x {inst.R} replicas x {sweeps} sweeps of "
-                   f"n={inst.n} on {threads} threads ({r['attempts']} attempts, {dt:.1f} s); "
-                   "reference floating-point roulette (long double exp2, double sums) and "
-                   "full n^2 force update per accepted move as CPU_PIM.cpp:195-290"),
-    }
+def host_info():
+    """Host cores usable by this process and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"nproc": usable, "cpu_count": os.cpu_count(), "cpu_model": model}
 
 
 def _pct(xs, q):
@@ -113,81 +76,411 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=float), q)) if len(xs) else None
 
 
-def _chunked_run(pool, sweeps, chunks, verbose=False):
-    """Runs `sweeps` in `chunks` equal pieces; returns the device-run clock after each
-    piece (host reads between pieces excluded) and every instance's best cost then."""
-    step = sweeps // chunks
-    clock, traj = 0.0, []
-    for c in range(chunks):
+def batch_instances(pkg, sweeps):
+    """The 512-instance batch: n uniform [30,150], alternating dense / sparse flow,
+    Euclidean distance, 32 replicas on T 100..10000, one exchange per sweep."""
+    I = pkg.instances
+    ladder = I.geometric_ladder(32, 100.0, 10000.0)
+    return [pkg.Instance(F=F, D=D, beta_ln=ladder, sweeps=sweeps, seed=7000 + k)
+            for k, (F, D) in enumerate(I.batch_c3())]
+
+
+def workload(pkg, name, sweeps=0):
+    """Returns (instances, description dict) of a --mode throughput workload."""
+    I = pkg.instances
+    if name == "grey256":
+        F, D = I.grey_c2()
+        betas = I.geometric_ladder(64, 10.0, 1000.0)
+        inst = [pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=sweeps, seed=2024)]
+        desc = {"workload": "grey-pattern n=256 (92x92 unit-flow block, squared torus "
+                            "Manhattan distance 16x16), 64 replicas, geometric T 10..1000, "
+                            "exchange every sweep", "n": 256, "replicas": 64, "instances": 1}
+        return inst, desc
+    if name == "flat64":
+        F, D = I.flat_c1()
+        betas = I.geometric_ladder(16, 1.0, 100.0)
+        inst = [pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=sweeps, seed=2024)]
+        desc = {"workload": "n=64 uniform flow [0,99], rounded Euclidean distance, 16 "
+                            "replicas, geometric T 1..100, exchange every sweep", "n": 64,
+                "replicas": 16, "instances": 1}
+        return inst, desc
+    if name == "batch512":
+        desc = {"workload": "512 instances, n uniform [30,150], half dense / half sparse "
+                            "flow, Euclidean distance, 32 replicas each, geometric T "
+                            "100..10000, exchange every sweep, one pool",
+                "n": "30..150", "replicas": 32 * 512, "instances": 512}
+        return batch_instances(pkg, sweeps), desc
+    if name in ("large512", "large1024"):
+        n = 512 if name == "large512" else 1024
+        F, D = I.large_c5(n)
+        betas = I.geometric_ladder(64, 300.0, 30000.0)
+        inst = [pkg.Instance(F=F, D=D, beta_ln=betas, sweeps=sweeps, seed=2024)]
+        desc = {"workload": f"n={n} flow U[0,255], Manhattan grid distance, 64 replicas, "
+                            "geometric T 300..30000, exchange every sweep", "n": n,
+                "replicas": 64, "instances": 1}
+        return inst, desc
+    raise SystemExit(f"unknown workload {name}")
+
+
+# ---------------------------------------------------------------------------------------
+# roofline of the dominant sweep kernel
+# ---------------------------------------------------------------------------------------
+def algorithmic_bytes(att, acc, cells, n):
+    """24 n bytes per attempt (six int32 n-vectors: force row, gathered force column,
+    gathered force diagonal, distance row, flow row, inverse permutation) plus, per
+    accepted move, 16 n bytes for the two difference vectors and 8 bytes (int32 read +
+    write) per force entry the move rewrites."""
+    return 24 * n * att + 16 * n * acc + 8 * cells
+
+
+def roofline_pass(pool, insts, sweeps, traffic_name):
+    """Runs `sweeps` more sweeps of `pool` in timed mode (every sweep launch bracketed by
+    HIP events on its own stream, buckets concurrent as in the graphs) and returns the
+    roofline object of the bucket with the most device time."""
+    before = [pool.result(k) for k in range(len(insts))]
+    pool.run(sweeps, timed=True)
+    after = [pool.result(k) for k in range(len(insts))]
+    shapes = [pool.shape(k) for k in range(len(insts))]
+    buckets = pool.buckets()
+    # the bucket with the most device time; buckets run concurrently, so another within 5 %
+    # of it is as much the critical path, and the one with more replicas is taken
+    top = max(b["kernel_ms"] for b in buckets)
+    best = max((b for b in buckets if b["kernel_ms"] >= 0.95 * top),
+               key=lambda b: (b["replicas"], b["kernel_ms"]))
+    key = ",".join(best["shape"].split(",")[:3])  # NTxCPL,placement,sym|general
+    alg = 0
+    members = 0
+    for k, x in enumerate(insts):
+        if ",".join(shapes[k].split(",")[:3]) != key:
+            continue
+        members += 1
+        alg += algorithmic_bytes(after[k]["attempts"] - before[k]["attempts"],
+                                 after[k]["accepted"] - before[k]["accepted"],
+                                 after[k]["update_cells"] - before[k]["update_cells"], x.n)
+    launches = max(best["launches"], 1)
+    avg_s = best["kernel_ms"] / 1e3 / launches
+    achieved = alg / launches / avg_s / 1e9
+    placement = key.split(",")[1]
+    lds = placement in ("lds32", "lds16")
+    peak = LDS_PEAK_GBS if lds else HBM_PEAK_GBS
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{traffic_name}.txt")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            rec = json.load(fh)
+        if rec.get("shape", best["shape"]) == best["shape"]:
+            traffic = rec.get("hbm_bytes_per_launch")
+    out = {
+        "bound": "lds" if lds else "hbm",
+        "achieved": achieved,
+        "peak": peak,
+        "unit": "GB/s",
+        "frac": achieved / peak,
+        "traffic": traffic,
+        "kernel": best["shape"],
+        "kernel_avg_us": avg_s * 1e6,
+        "kernel_launches": best["launches"],
+        "instances_in_kernel": members,
+        "algorithmic_bytes_per_launch": alg / launches,
+        "peak_basis": ("LDS: 256 CUs x 128 B/clk x 2.4 GHz" if lds else "HBM3E 8 TB/s"),
+    }
+    if traffic is not None:  # counter HBM traffic against the HBM roof, beside the bound
+        out["hbm_traffic_gbs"] = traffic / avg_s / 1e9
+        out["hbm_traffic_frac"] = traffic / avg_s / 1e9 / HBM_PEAK_GBS
+    out["buckets"] = [{"shape": b["shape"], "replicas": b["replicas"],
+                       "avg_us": b["kernel_ms"] * 1e3 / max(b["launches"], 1)}
+                      for b in buckets]
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+# CPU baseline: the oracle's reference-arithmetic mode (test infrastructure, timed here)
+# ---------------------------------------------------------------------------------------
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    return pyoracle
+
+
+def cpu_reference_runs(insts, sample, sweeps, threads):
+    """One run of the oracle's reference-arithmetic mode (CPU_PIM.cpp:195-290: long double
+    exp2 weights, double sums, full n^2 force update per accepted move) per sampled
+    instance at `sweeps` sweeps, `threads` at a time.  Returns per-instance best costs
+    and seconds."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    po = _oracle()
+
+    def one(k):
+        x = insts[k]
         t0 = time.perf_counter()
-        pool.run(step)
-        clock += time.perf_counter() - t0
-        traj.append((clock, pool.best_costs()))
-        if verbose and (c + 1) % 10 == 0:
-            print(f"#   {c + 1}/{chunks} pieces, {clock:.1f} s", flush=True)
+        r = po.solve(x.F, x.D, x.beta_ln, sweeps, seed=x.seed, mode=po.MODE_REFFP)
+        return k, r["best_cost"], r["attempts"], time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        rows = list(ex.map(one, sample))
+    return rows, time.perf_counter() - t0
+
+
+def cpu_rate_samples(insts, sample, sweeps, threads, repeats=5):
+    """Throughput of the reference arithmetic on `threads` host cores: `repeats` runs of
+    every sampled instance for `sweeps` sweeps, one instance per thread; each run's rate
+    is threads x (attempts / thread-seconds) summed over the instances, i.e. what the cores
+    sustain while all are busy.  Returns the median and every run's rate."""
+    rates = []
+    for _ in range(repeats):
+        rows, _ = cpu_reference_runs(insts, sample, sweeps, threads)
+        att = sum(r[2] for r in rows)
+        busy = sum(r[3] for r in rows)
+        rates.append(min(threads, len(sample)) * att / busy)
+    rates.sort()
+    return rates[len(rates) // 2], rates
+
+
+def cpu_baseline_single(inst, sweeps, threads, repeats=5):
+    """--mode throughput: `threads` independent ladders of the workload's instance (the
+    reference's -t model), `sweeps` sweeps each, oracle reference arithmetic; median of
+    `repeats` runs."""
+    po = _oracle()
+    rates, att, dt = [], 0, 0.0
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        r = po.solve(inst.F, inst.D, inst.beta_ln, sweeps, T=threads, seed=inst.seed,
+                     mode=po.MODE_REFFP, nthreads=threads)
+        dt = time.perf_counter() - t0
+        att = r["attempts"]
+        rates.append(att / dt)
+    rates.sort()
+    return rates[len(rates) // 2], rates, att, dt
+
+
+# ---------------------------------------------------------------------------------------
+# default: batched vs plain on the 512-instance batch
+# ---------------------------------------------------------------------------------------
+def _stepped(pool, steps, sps, k_of=None):
+    """K steps of `sps` sweeps; returns [(seconds since start, best costs)] after each."""
+    traj = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pool.run(sps)
+        t = time.perf_counter() - t0
+        traj.append((t, pool.best_costs()))
     return traj
 
 
-def _tts(traj, k):
-    """Time at which instance k first came within 1 % of its best cost at the full sweep
-    budget.  The trajectory is bit-identical to the CPU restatement's at equal sweeps,
-    so that best cost is also the CPU run's."""
-    final = int(traj[-1][1][k])
-    target = final + abs(final) // 100
-    return next(t for t, best in traj if int(best[k]) <= target)
+def batched(pkg, args):
+    import numpy as np
 
+    K, W = args.steps, args.warmup
+    S = args.sweeps_per_step or 10
+    total = K * S
+    insts = batch_instances(pkg, total)
+    N = len(insts)
+    ns = np.array([x.n for x in insts])
+    stride = max(1, N // args.plain_sample)
+    sample = list(range(0, N, stride))[:args.plain_sample]
+    print(f"# batched: {N} instances x 32 replicas, {K} steps x {S} sweeps, "
+          f"plain sample {len(sample)}", flush=True)
 
-def batch_vs_plain(pkg, args):
-    """512 instances (n 30..150, half dense / half sparse flow, 32 replicas each) in one
-    pool, against the plain path (one instance per pool, run one after another) on an
-    evenly spaced sample; both report swap-attempts/s and time-to-target percentiles."""
-    I = pkg.instances
-    sweeps, chunks = args.tts_sweeps, args.tts_chunks
-    insts = [pkg.Instance(F=F, D=D, beta_ln=I.geometric_ladder(32, 100.0, 10000.0),
-                          sweeps=sweeps, seed=7000 + k)
-             for k, (F, D) in enumerate(I.batch_c3())]
+    if W:  # warm-up on a separate pool of the same instances (kernels loaded, attributes set)
+        with pkg.Pool(insts) as wp:
+            for _ in range(W):
+                wp.run(S)
+
+    # ---- batched: one pool, K steps through the graph path ---------------------------
     pool = pkg.Pool(insts)
-    print(f"# batched: {len(insts)} instances, {sweeps} sweeps in {chunks} pieces", flush=True)
-    traj = _chunked_run(pool, sweeps, chunks, verbose=True)
-    att_b = sum(pool.result(k)["attempts"] for k in range(len(insts)))
+    traj_b = _stepped(pool, K, S)
+    wall_b = traj_b[-1][0]
+    res_b = [pool.result(k) for k in range(N)]
+    att_b = sum(r["attempts"] for r in res_b)
+    print(f"# batched: {wall_b:.3f} s, {att_b / wall_b / 1e6:.1f} M swap-attempts/s", flush=True)
+    roof = roofline_pass(pool, insts, args.roofline_steps * S, "batch512")
     fm, cm = pool.verify()
     pool.close()
-    t_b = traj[-1][0]
-    tts_b = [_tts(traj, k) for k in range(len(insts))]
 
-    sample = list(range(0, len(insts), max(1, len(insts) // args.plain_sample)))[:args.plain_sample]
-    t_p, att_p, tts_p, same = 0.0, 0, [], True
+    # ---- plain: one instance per pool, one after another -----------------------------
+    plain_t, plain_att, traj_p, same = [], 0, {}, True
     for k in sample:
         with pkg.Pool([insts[k]]) as one:
-            tr = _chunked_run(one, sweeps, chunks)
-            att_p += one.result(0)["attempts"]
-        t_p += tr[-1][0]
-        tts_p.append(_tts(tr, 0))
-        same &= all(int(b1[0]) == int(b2[k]) for (_, b1), (_, b2) in zip(tr, traj))
-        print(f"# plain instance {k} (n={insts[k].n}): {tr[-1][0]:.2f} s", flush=True)
-    rate_b, rate_p = att_b / t_b, att_p / t_p
-    return {
-        "metric": "swap-attempts/s; p50/p99 time-to-target (batched vs plain)",
+            tr = _stepped(one, K, S)
+            plain_att += one.result(0)["attempts"]
+        plain_t.append(tr[-1][0])
+        traj_p[k] = tr
+        same &= all(int(b1[0]) == int(b2[k]) for (_, b1), (_, b2) in zip(tr, traj_b))
+    wall_p = sum(plain_t)
+    rate_b, rate_p = att_b / wall_b, plain_att / wall_p
+    print(f"# plain: {len(sample)} instances, {wall_p:.3f} s, {rate_p / 1e6:.2f} M/s; "
+          f"trajectories identical: {same}", flush=True)
+
+    # queue model of the plain path: each instance's run time from a least-squares line in
+    # its size over the sample, summed over the instances ahead of it in the queue
+    A = np.vstack([np.ones(len(sample)), ns[sample]]).T
+    coef, *_ = np.linalg.lstsq(A, np.array(plain_t), rcond=None)
+    pred = np.maximum(coef[0] + coef[1] * ns, 0.0)
+    ahead = np.concatenate([[0.0], np.cumsum(pred)[:-1]])
+
+    # ---- CPU reference runs: targets at equal sweeps, and the CPU throughput ---------
+    host = host_info()
+    threads = args.cpu_threads or max(1, min(16, host["nproc"]))
+    cpu = None
+    if not args.no_cpu:
+        rows, cpu_wall = cpu_reference_runs(insts, sample, total, threads)
+        ref_best = {k: b for k, b, _, _ in rows}
+        short = max(1, total // 10)
+        rate_med, rates = cpu_rate_samples(insts, sample, short, threads, repeats=5)
+        cpu = {
+            "value": rate_med,
+            "unit": "swap-attempts/s",
+            "cores": min(threads, len(sample)),
+            "kind": "port",
+            "sample": (f"oracle reference-arithmetic mode (CPU_PIM.cpp:195-290: long double "
+                       f"exp2 roulette, double sums, full n^2 force update per accepted move) on "
+                       f"the {len(sample)} sampled instances, one per thread on {threads} "
+                       f"threads, {short} sweeps each; median of 5 runs, each run's rate = "
+                       f"cores x attempts / busy thread-seconds"),
+            "runs": rates,
+            "host": host,
+            "target_runs": {"sweeps": total, "wall_s": cpu_wall,
+                            "attempts": int(sum(r[2] for r in rows))},
+        }
+        base = ref_best
+        target_source = (f"best cost of the oracle's reference-arithmetic run (CPU_PIM.cpp:"
+                         f"195-290 roulette) at {total} sweeps, per sampled instance")
+    else:
+        base = {k: int(traj_b[-1][1][k]) for k in sample}
+        target_source = "the batched run's own final best (--no-cpu)"
+        ref_best = {}
+
+    def tts(tol_per_mille):
+        """Time-to-target percentiles with target = base x (1 + tol/1000), both paths."""
+        target = {k: base[k] + abs(base[k]) * tol_per_mille // 1000 for k in sample}
+
+        def first(traj, k, idx):
+            return next((t for t, b in traj if int(b[idx]) <= target[k]), None)
+
+        tb_all, tp_all, tp_own = [], [], []
+        for k in sample:
+            tb = first(traj_b, k, k)
+            if tb is not None:
+                tb_all.append(tb)
+            tp = first(traj_p[k], k, 0)
+            if tp is not None:
+                tp_own.append(tp)
+                tp_all.append(float(ahead[k]) + tp)
+        return {"batched": {"p50_s": _pct(tb_all, 50), "p99_s": _pct(tb_all, 99),
+                            "reached": len(tb_all)},
+                "plain": {"p50_s": _pct(tp_all, 50), "p99_s": _pct(tp_all, 99),
+                          "reached": len(tp_all),
+                          "own_p50_s": _pct(tp_own, 50), "own_p99_s": _pct(tp_own, 99)}}
+
+    t1, t01 = tts(10), tts(1)
+    gpu_vs_cpu = None
+    if ref_best:
+        gpu_vs_cpu = sum(int(traj_b[-1][1][k]) <= ref_best[k] for k in sample)
+
+    out = {
+        "metric": METRIC,
         "value": rate_b,
         "unit": "swap-attempts/s",
         "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall_b * 1e3 / K,
         "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "512 instances, n uniform [30,150], half dense / half sparse "
-                               "flow, Euclidean distance, 32 replicas each, geometric T "
-                               "100..10000, exchange every sweep", "sweeps": sweeps,
-                   "pieces": chunks, "plain_sample": len(sample)},
-        "batched": {"attempts_per_s": rate_b, "seconds": t_b,
-                    "tts_p50_s": _pct(tts_b, 50), "tts_p99_s": _pct(tts_b, 99),
-                    "verify_mismatches": fm + cm},
-        "plain": {"attempts_per_s": rate_p, "seconds_sample": t_p,
-                  "tts_p50_s": _pct(tts_p, 50), "tts_p99_s": _pct(tts_p, 99),
-                  "tts_note": "each instance on its own clock (no queueing counted)"},
+        "config": {"workload": f"batched: 512 instances, n uniform [30,150], half dense / half "
+                               f"sparse flow, rounded Euclidean distance, 32 replicas each on "
+                               f"geometric T 100..10000, exchange every sweep, {total} sweeps "
+                               f"in one pool (hipGraph-captured sweep+exchange intervals)",
+                   "instances": N, "replicas_per_instance": 32, "n": "30..150",
+                   "sweeps_per_step": S, "sweeps": total, "plain_sample": len(sample)},
         "batched_over_plain": rate_b / rate_p,
+        "batched": {"swap_attempts_per_s": rate_b, "seconds": wall_b},
+        "plain": {"swap_attempts_per_s": rate_p, "seconds_sample": wall_p,
+                  "instances": len(sample)},
+        "time_to_target": {
+            "target": "best cost <= 1.01 x the reference best",
+            "target_source": target_source,
+            "instances": len(sample),
+            "batched": t1["batched"],
+            "plain": t1["plain"],
+            "tight_0.1pct": t01,
+            "clock": ("batched: from the start of the batch; plain: from the start of the "
+                      "queue, i.e. the instance's own time plus the run times of the "
+                      "instances ahead of it, each modelled as a + b n from the sample's "
+                      "measured run times (own_* = the instance's own clock); checkpoints "
+                      f"every {S} sweeps"),
+            "gpu_best_le_cpu_best": gpu_vs_cpu,
+        },
         "trajectories_identical": bool(same),
-        "target": "best cost within 1% of the best at the full sweep budget",
+        "verify_mismatches": fm + cm,
+        "roofline": roof,
     }
+    if cpu:
+        out["cpu_baseline"] = cpu
+    return out
+
+
+def throughput(pkg, args):
+    """One workload's throughput line (--mode throughput)."""
+    sps = args.sweeps_per_step or {"grey256": 100, "flat64": 200, "batch512": 10,
+                                   "large512": 10, "large1024": 4}[args.workload]
+    insts, desc = workload(pkg, args.workload)
+    if args.dynamics == "one-shot":
+        for x in insts:
+            x.one_shot = True
+        desc = dict(desc, dynamics="one-shot parallel proposals (a step = n proposals)")
+    if args.warmup:
+        with pkg.Pool(insts) as wp:
+            wp.run(args.warmup * sps)
+    pool = pkg.Pool(insts)
+    shape = pool.shape(0)
+    traj = _stepped(pool, args.steps, sps)
+    wall = traj[-1][0]
+    res = [pool.result(k) for k in range(len(insts))]
+    attempts = sum(r["attempts"] for r in res)
+    acc = sum(r["accepted"] for r in res)
+    suffix = "_oneshot" if args.dynamics == "one-shot" else ""
+    roof = roofline_pass(pool, insts, args.roofline_steps * sps, args.workload + suffix)
+    fm, cm = pool.verify()
+    pool.close()
+    out = {
+        "metric": "swap-attempts/s",
+        "value": attempts / wall,
+        "unit": "swap-attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": dict(desc, sweeps_per_step=sps, kernel_shape=shape),
+        "roofline": roof,
+        "accept_rate": acc / max(attempts, 1),
+        "verify_mismatches": fm + cm,
+        "best_cost": min(r["best_cost"] for r in res),
+    }
+    if not args.no_cpu and args.dynamics == "pdit" and len(insts) == 1:
+        host = host_info()
+        threads = args.cpu_threads or max(1, min(16, host["nproc"]))
+        sweeps = {"grey256": 1, "flat64": 400, "large512": 1, "large1024": 1}[args.workload]
+        med, rates, att, dt = cpu_baseline_single(insts[0], sweeps, threads)
+        out["cpu_baseline"] = {
+            "value": med, "unit": "swap-attempts/s", "cores": threads, "kind": "port",
+            "sample": (f"{threads} independent ladders x {insts[0].R} replicas x {sweeps} "
+                       f"sweeps of n={insts[0].n} on {threads} threads ({att} attempts, "
+                       f"{dt:.1f} s per run), median of 5 runs; oracle reference-arithmetic "
+                       f"mode (CPU_PIM.cpp:195-290)"),
+            "runs": rates, "host": host}
+    return out
 
 
 def dynamics_tts(pkg, args):
@@ -196,7 +489,7 @@ def dynamics_tts(pkg, args):
     of steps until it has used as much device time (at most 20x the pieces).  Target: the
     p-dit chain's best cost at the full sweep budget + 1 % (that chain is bit-identical to
     the CPU restatement, so this is also the CPU run's best at equal sweeps)."""
-    insts, desc, _ = workload(pkg, args.workload)
+    insts, desc = workload(pkg, args.workload)
     sweeps, chunks = args.tts_sweeps, args.tts_chunks
 
     def run(one_shot, budget_s=None):
@@ -333,23 +626,27 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="batched",
+                    choices=["batched", "throughput", "served", "dynamics"],
+                    help="batched (default): the 512-instance batch against the plain path, "
+                         "with time-to-target percentiles; throughput: the one-line bench of "
+                         "--workload; served: Poisson request streams through the serving "
+                         "loop; dynamics: the one-shot chain against the p-dit chain")
     ap.add_argument("--workload", default="grey256",
-                    choices=["grey256", "flat64", "batch512", "large512", "large1024"])
+                    choices=["grey256", "flat64", "batch512", "large512", "large1024"],
+                    help="--mode throughput / dynamics workload")
     ap.add_argument("--sweeps-per-step", type=int, default=0)
+    ap.add_argument("--roofline-steps", type=int, default=3,
+                    help="steps of the timed pass that measures the dominant kernel")
     ap.add_argument("--dynamics", default="pdit", choices=["pdit", "one-shot"],
                     help="pdit: sequential p-dit sweeps (CPU_PIM's chain, the default); "
                          "one-shot: parallel-proposal steps (GPU-CUPIM's driver), a step "
                          "counts as a sweep of n attempts")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", default="throughput",
-                    choices=["throughput", "batch-vs-plain", "served", "dynamics"],
-                    help="throughput: the one-line bench of --workload; batch-vs-plain and "
-                         "served: time-to-target / latency percentiles against the plain path; "
-                         "dynamics: time-to-target of the one-shot chain against the p-dit chain")
+    ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--tts-sweeps", type=int, default=10000)
     ap.add_argument("--tts-chunks", type=int, default=50)
-    ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--served-sweeps", type=int, default=10000)
     ap.add_argument("--rates", default="10,100,1000")
@@ -384,86 +681,20 @@ def main():
                                    "independent and would shard by instance across GPUs, "
                                    "but only the one-GPU path is built; rerun with --gpus 1"}))
         sys.exit(2)
+    if args.steps < 1:
+        raise SystemExit("--steps must be >= 1")
 
     pkg = load_pkg()
-    if args.mode == "batch-vs-plain":
-        print(json.dumps(batch_vs_plain(pkg, args)))
-        return
+    if pkg.device_count() < 1:
+        raise SystemExit("no HIP device visible to libqap_pt.so")
     if args.mode == "served":
         print(json.dumps(served(pkg, args)))
-        return
-    if args.mode == "dynamics":
+    elif args.mode == "dynamics":
         print(json.dumps(dynamics_tts(pkg, args)))
-        return
-    insts, desc, cpu_spec = workload(pkg, args.workload)
-    if args.dynamics == "one-shot":
-        for x in insts:
-            x.one_shot = True
-        desc = dict(desc, dynamics="one-shot parallel proposals (a step = n proposals)")
-    sps = args.sweeps_per_step or {"grey256": 100, "flat64": 200, "batch512": 10,
-                                   "large512": 10, "large1024": 4}[args.workload]
-
-    pool = pkg.Pool(insts)
-    shape = pool.shape(0)
-    if args.warmup:
-        pool.run(args.warmup * sps)
-    before = [pool.result(k) for k in range(len(insts))]
-    t0 = time.perf_counter()
-    kernel_ms, launches = pool.run(args.steps * sps, timed=True)
-    t1 = time.perf_counter()
-    after = [pool.result(k) for k in range(len(insts))]
-    fm, cm = pool.verify()
-    pool.close()
-
-    wall = t1 - t0
-    attempts = sum(a["attempts"] - b["attempts"] for a, b in zip(after, before))
-    n_att = sum((a["attempts"] - b["attempts"]) * x.n for a, b, x in zip(after, before, insts))
-    acc = sum(a["accepted"] - b["accepted"] for a, b in zip(after, before))
-    acc_n = sum((a["accepted"] - b["accepted"]) * x.n for a, b, x in zip(after, before, insts))
-    cells = sum(a["update_cells"] - b["update_cells"] for a, b in zip(after, before))
-    alg = algorithmic_bytes(attempts, n_att, acc_n, cells)
-    avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
-    achieved = alg / max(launches, 1) / avg_launch_s / 1e9
-
-    traffic = None
-    suffix = "_oneshot" if args.dynamics == "one-shot" else ""
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{args.workload}{suffix}.txt")
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
-
-    out = {
-        "metric": "swap-attempts/s",
-        "value": attempts / wall,
-        "unit": "swap-attempts/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": dict(desc, sweeps_per_step=sps, kernel_shape=shape),
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-        },
-        "accept_rate": acc / max(attempts, 1),
-        "sweep_kernel_avg_ms": kernel_ms / max(launches, 1),
-        "sweep_kernel_launches": launches,
-        "verify_mismatches": fm + cm,
-        "best_cost": min(a["best_cost"] for a in after),
-    }
-    if not args.no_cpu and args.dynamics == "pdit":
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(pkg, insts, cpu_spec, threads)
-    print(json.dumps(out))
+    elif args.mode == "throughput":
+        print(json.dumps(throughput(pkg, args)))
+    else:
+        print(json.dumps(batched(pkg, args)))
 
 
 if __name__ == "__main__":

@@ -226,6 +226,15 @@ int qap_pt_kernel_shape(int32_t n, int64_t replicas, char *buf, int32_t buflen);
 /* Shape actually used for instance inst of a pool ("NTxCPL,placement,sym|general,
  * dense-flow|sparse-flow"). */
 int qap_pt_pool_shape(qap_pt_pool *pool, int32_t inst, char *buf, int32_t buflen);
+/* Kernel bucket b of a pool (the replicas of one sweep-kernel instantiation, launched
+ * together every exchange interval; b = 0, 1, ... until QAP_PT_E_INVAL): its shape
+ * ("NTxCPL,placement,sym|general[,sparse-flow]", sparse-flow when the kernel is built
+ * without the dense update path), its replicas, and the summed device time and number of
+ * its sweep launches over the pool's timed runs (qap_pt_pool_run with kernel_ms: HIP
+ * events on the stream each launch goes to, buckets concurrent as in the graphs).  A
+ * report with no counterpart in the reference; each pointer may be NULL. */
+int qap_pt_pool_bucket(qap_pt_pool *pool, int32_t b, char *shape, int32_t buflen,
+                       int64_t *replicas, double *kernel_ms, int64_t *launches);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,7 @@ EXPORTS = (
     "qap_pt_pool_verify",
     "qap_pt_pool_destroy",
     "qap_pt_kernel_shape",
+    "qap_pt_pool_bucket",
     "qap_pt_pool_shape",
     "qap_pt_auto_beta",
     "qap_pt_auto_select",
@@ -152,6 +153,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.qap_pt_pool_destroy.argtypes = [C.c_void_p]
     lib.qap_pt_kernel_shape.argtypes = [C.c_int32, C.c_int64, C.c_char_p, C.c_int32]
     lib.qap_pt_pool_shape.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]
+    lib.qap_pt_pool_bucket.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, _I64P,
+                                       _F64P, _I64P]
     lib.qap_pt_auto_beta.argtypes = [C.POINTER(QapProblem), C.POINTER(QapPtParams), C.c_double,
                                      C.c_double, _F64P, _F64P, _F64P]
     lib.qap_pt_auto_select.argtypes = [_F64P, C.c_int32, C.c_double, C.c_double, C.c_int32,
@@ -387,6 +390,18 @@ class Pool:
         buf = C.create_string_buffer(96)
         _check(self.lib.qap_pt_pool_shape(self.handle, k, buf, 96))
         return buf.value.decode()
+
+    def buckets(self) -> list[dict]:
+        """Kernel buckets: shape, replicas, and device time / launches of timed runs."""
+        out, b = [], 0
+        buf = C.create_string_buffer(96)
+        reps, ms, nl = C.c_int64(0), C.c_double(0.0), C.c_int64(0)
+        while self.lib.qap_pt_pool_bucket(self.handle, b, buf, 96, C.byref(reps), C.byref(ms),
+                                          C.byref(nl)) == QAP_PT_OK:
+            out.append({"shape": buf.value.decode(), "replicas": int(reps.value),
+                        "kernel_ms": float(ms.value), "launches": int(nl.value)})
+            b += 1
+        return out
 
     def verify(self) -> tuple[int, int]:
         fm = C.c_int64(0)

@@ -310,8 +310,10 @@ void pool_free(qap_pt_pool *p) {
     if (!p) return;
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
-    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (auto &g : p->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
     if (p->arena) {  // nothing here was allocated by the pool
         delete p;
         return;
@@ -336,18 +338,8 @@ void pool_free(qap_pt_pool *p) {
 namespace {
 
 
-// One graph = `depth` mega-iterations: exchange, then every sweep bucket.  Buckets
-// (different kernel instantiations) run concurrently on side streams that fork from
-// and join back into the pool stream.
-int build_graph(qap_pt_pool *p, int depth) {
-    if (p->graph_exec) {
-        (void)hipGraphExecDestroy(p->graph_exec);
-        p->graph_exec = nullptr;
-    }
-    if (p->graph) {
-        (void)hipGraphDestroy(p->graph);
-        p->graph = nullptr;
-    }
+// Side streams and fork/join events for buckets beyond the first (created once per pool).
+int ensure_side_streams(qap_pt_pool *p) {
     const int nb = (int)p->buckets.size();
     while ((int)p->side.size() < nb - 1) {
         hipStream_t st = nullptr;
@@ -358,25 +350,54 @@ int build_graph(qap_pt_pool *p, int depth) {
         p->ev_join.push_back(ev);
     }
     if (nb > 1 && !p->ev_fork) HIP_TRY(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+    return QAP_PT_OK;
+}
+
+// One exchange interval: exchange, then every sweep bucket.  Buckets (different kernel
+// instantiations) run concurrently on side streams that fork from and join back into the
+// pool stream.  With `ev` (timed runs) each bucket's launch is bracketed by
+// ev[2 b], ev[2 b + 1] recorded on the stream it is launched on.
+hipError_t enqueue_interval(qap_pt_pool *p, const hipEvent_t *ev) {
+    const int nb = (int)p->buckets.size();
+    hipError_t e = launch_exchange(p->P, p->count, p->max_rt, p->stream);
+    if (e == hipSuccess && nb > 1) e = hipEventRecord(p->ev_fork, p->stream);
+    auto sweep = [&](int b, hipStream_t st) {
+        const Bucket &B = p->buckets[b];
+        if (e == hipSuccess && ev) e = hipEventRecord(ev[2 * b], st);
+        if (e == hipSuccess)
+            e = launch_sweep(p->P, B.shape, B.d_list, (int)B.reps.size(), B.lds_bytes, st);
+        if (e == hipSuccess && ev) e = hipEventRecord(ev[2 * b + 1], st);
+    };
+    for (int b = 1; b < nb && e == hipSuccess; ++b) {
+        e = hipStreamWaitEvent(p->side[b - 1], p->ev_fork, 0);
+        sweep(b, p->side[b - 1]);
+        if (e == hipSuccess) e = hipEventRecord(p->ev_join[b - 1], p->side[b - 1]);
+    }
+    if (nb > 0) sweep(0, p->stream);
+    for (int b = 1; b < nb && e == hipSuccess; ++b)
+        e = hipStreamWaitEvent(p->stream, p->ev_join[b - 1], 0);
+    return e;
+}
+
+// The executable graph of `depth` exchange intervals, captured on first use.  At most four
+// depths are kept (a pool runs one or two: its step and a remainder).
+int graph_for(qap_pt_pool *p, int depth, hipGraphExec_t *out) {
+    for (auto &g : p->graphs)
+        if (g.depth == depth) {
+            *out = g.exec;
+            return QAP_PT_OK;
+        }
+    if (p->graphs.size() >= 4) {
+        (void)hipStreamSynchronize(p->stream);
+        (void)hipGraphExecDestroy(p->graphs.front().exec);
+        (void)hipGraphDestroy(p->graphs.front().graph);
+        p->graphs.erase(p->graphs.begin());
+    }
+    const int rc = ensure_side_streams(p);
+    if (rc) return rc;
     HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
-    for (int m = 0; m < depth && e == hipSuccess; ++m) {
-        e = launch_exchange(p->P, p->count, p->max_rt, p->stream);
-        if (e == hipSuccess && nb > 1) e = hipEventRecord(p->ev_fork, p->stream);
-        for (int b = 1; b < nb && e == hipSuccess; ++b) {
-            e = hipStreamWaitEvent(p->side[b - 1], p->ev_fork, 0);
-            if (e == hipSuccess)
-                e = launch_sweep(p->P, p->buckets[b].shape, p->buckets[b].d_list,
-                                 (int)p->buckets[b].reps.size(), p->buckets[b].lds_bytes,
-                                 p->side[b - 1]);
-            if (e == hipSuccess) e = hipEventRecord(p->ev_join[b - 1], p->side[b - 1]);
-        }
-        if (e == hipSuccess && nb > 0)
-            e = launch_sweep(p->P, p->buckets[0].shape, p->buckets[0].d_list,
-                             (int)p->buckets[0].reps.size(), p->buckets[0].lds_bytes, p->stream);
-        for (int b = 1; b < nb && e == hipSuccess; ++b)
-            e = hipStreamWaitEvent(p->stream, p->ev_join[b - 1], 0);
-    }
+    for (int m = 0; m < depth && e == hipSuccess; ++m) e = enqueue_interval(p, nullptr);
     if (e != hipSuccess) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(p->stream, &g);
@@ -384,9 +405,41 @@ int build_graph(qap_pt_pool *p, int depth) {
         return fail(QAP_PT_E_DEVICE, "kernel launch during capture failed: %s",
                     hipGetErrorString(e));
     }
-    HIP_TRY(hipStreamEndCapture(p->stream, &p->graph));
-    HIP_TRY(hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0));
-    p->graph_depth = depth;
+    qap_pt_pool::CapturedRun g{depth, nullptr, nullptr};
+    HIP_TRY(hipStreamEndCapture(p->stream, &g.graph));
+    e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g.graph);
+        return fail(QAP_PT_E_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    p->graphs.push_back(g);
+    *out = g.exec;
+    return QAP_PT_OK;
+}
+
+// Enqueues `intervals` exchange intervals on the pool stream: replays of the deepest graph
+// (up to kGraphDepth intervals) and one of the remainder, or, for an arena pool, the same
+// launches one by one.
+constexpr int kGraphDepth = 32;
+int enqueue_intervals(qap_pt_pool *p, int64_t intervals) {
+    if (p->plain_launch) {
+        // no side streams, events or graph objects: every bucket in turn on the pool stream
+        for (int64_t x = 0; x < intervals; ++x) {
+            HIP_TRY(launch_exchange(p->P, p->count, p->max_rt, p->stream));
+            for (auto &b : p->buckets)
+                HIP_TRY(launch_sweep(p->P, b.shape, b.d_list, (int)b.reps.size(), b.lds_bytes,
+                                     p->stream));
+        }
+        return QAP_PT_OK;
+    }
+    for (int64_t m = 0; m < intervals;) {
+        const int depth = (int)std::min<int64_t>(kGraphDepth, intervals - m);
+        hipGraphExec_t ex = nullptr;
+        const int rc = graph_for(p, depth, &ex);
+        if (rc) return rc;
+        HIP_TRY(hipGraphLaunch(ex, p->stream));
+        m += depth;
+    }
     return QAP_PT_OK;
 }
 
@@ -784,71 +837,60 @@ int pool_run_impl(qap_pt_pool *p, int64_t sweeps, double *kernel_ms, int64_t *la
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     };
     if (kernel_ms) {
-        // Timed mode: every sweep bucket launch bracketed by events on the pool stream.
-        const int chunk = 256;
-        std::vector<hipEvent_t> ev(2 * chunk * std::max<size_t>(p->buckets.size(), 1));
-        for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+        // Timed mode: the graph's launches issued directly, bucket concurrency included,
+        // every sweep launch bracketed by events on the stream it is launched on.
+        if (p->plain_launch) return fail(QAP_PT_E_INVAL, "timed runs need a pool of its own");
+        const int rc0 = ensure_side_streams(p);
+        if (rc0) return rc0;
+        const int nb = (int)p->buckets.size();
+        p->bucket_ms.resize(nb, 0.0);
+        p->bucket_launches.resize(nb, 0);
+        constexpr int chunk = 64;  // intervals between host reads of the events
+        std::vector<hipEvent_t> ev(2 * chunk * std::max(nb, 1), nullptr);
+        int rc = QAP_PT_OK;
+        for (auto &e : ev)
+            if (hipEventCreate(&e) != hipSuccess) rc = fail(QAP_PT_E_DEVICE, "hipEventCreate failed");
         double total = 0.0;
         int64_t nl = 0;
         int64_t m = 0;
-        int rc = QAP_PT_OK;
         while (m < max_new && rc == QAP_PT_OK) {
             int used = 0;
-            for (int c = 0; c < chunk && m < max_new; ++c, ++m) {
-                if (launch_exchange(p->P, p->count, p->max_rt, p->stream) != hipSuccess) {
-                    rc = fail(QAP_PT_E_DEVICE, "exchange launch failed");
+            for (; used < chunk && m < max_new; ++used, ++m) {
+                const hipError_t e = enqueue_interval(p, ev.data() + 2 * used * nb);
+                if (e != hipSuccess) {
+                    rc = fail(QAP_PT_E_DEVICE, "interval launch failed: %s", hipGetErrorString(e));
                     break;
                 }
-                for (auto &b : p->buckets) {
-                    (void)hipEventRecord(ev[2 * used], p->stream);
-                    if (launch_sweep(p->P, b.shape, b.d_list, (int)b.reps.size(), b.lds_bytes,
-                                     p->stream) != hipSuccess) {
-                        rc = fail(QAP_PT_E_DEVICE, "sweep launch failed");
-                        break;
-                    }
-                    (void)hipEventRecord(ev[2 * used + 1], p->stream);
-                    ++used;
-                }
-                if (rc) break;
             }
             if (hipStreamSynchronize(p->stream) != hipSuccess) {
                 rc = fail(QAP_PT_E_DEVICE, "stream synchronize failed");
                 break;
             }
-            for (int u = 0; u < used; ++u) {
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, ev[2 * u], ev[2 * u + 1]);
-                total += ms;
-            }
-            nl += used;
+            for (int u = 0; u < used; ++u)
+                for (int b = 0; b < nb; ++b) {
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, ev[2 * (u * nb + b)], ev[2 * (u * nb + b) + 1]);
+                    total += ms;
+                    p->bucket_ms[b] += ms;
+                    p->bucket_launches[b] += 1;
+                }
+            nl += (int64_t)used * nb;
             if (timeout > 0 && elapsed() > timeout) {
                 (void)launch_halt(p->d_state, p->count, p->stream);
                 break;
             }
         }
-        for (auto &e : ev) (void)hipEventDestroy(e);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
         if (rc) return rc;
         *kernel_ms = total;
         if (launches) *launches = nl;
     } else {
-        const int depth = (int)std::max<int64_t>(1, std::min<int64_t>(max_new, 32));
-        if (!p->plain_launch && max_new > 0 && (!p->graph_exec || p->graph_depth != depth)) {
-            const int rc = build_graph(p, depth);
+        // goal or timeout: the host looks at the instances after every kGraphDepth intervals
+        const int64_t batch = (any_goal || timeout > 0) ? kGraphDepth : std::max<int64_t>(max_new, 1);
+        for (int64_t m = 0; m < max_new; m += batch) {
+            const int rc = enqueue_intervals(p, std::min<int64_t>(batch, max_new - m));
             if (rc) return rc;
-        }
-        for (int64_t m = 0; m < max_new; m += depth) {
-            if (p->plain_launch) {
-                // the graph's launches issued one by one on the pool stream: exchange,
-                // then every bucket in turn (no side streams, events or graph objects)
-                for (int64_t x = m; x < std::min<int64_t>(max_new, m + depth); ++x) {
-                    HIP_TRY(launch_exchange(p->P, p->count, p->max_rt, p->stream));
-                    for (auto &b : p->buckets)
-                        HIP_TRY(launch_sweep(p->P, b.shape, b.d_list, (int)b.reps.size(),
-                                             b.lds_bytes, p->stream));
-                }
-            } else {
-                HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
-            }
             if (any_goal || timeout > 0) {
                 HIP_TRY(hipStreamSynchronize(p->stream));
                 if (timeout > 0 && elapsed() > timeout) {
@@ -945,13 +987,8 @@ int pool_launch_async(qap_pt_pool *p, hipEvent_t done) {
                            hipMemcpyHostToDevice, p->stream));
     HIP_TRY(launch_set_limits(p->d_state, p->d_limits, p->count, p->stream));
     if (max_new > 0) {
-        const int depth = (int)std::max<int64_t>(1, std::min<int64_t>(max_new, 32));
-        if (!p->graph_exec || p->graph_depth != depth) {
-            const int rc = build_graph(p, depth);
-            if (rc) return rc;
-        }
-        for (int64_t m = 0; m < max_new; m += p->graph_depth)
-            HIP_TRY(hipGraphLaunch(p->graph_exec, p->stream));
+        const int rc = enqueue_intervals(p, max_new);
+        if (rc) return rc;
     }
     HIP_TRY(hipEventRecord(done, p->stream));
     return QAP_PT_OK;
@@ -1147,6 +1184,23 @@ int qap_pt_pool_shape(qap_pt_pool *p, int32_t inst, char *buf, int32_t buflen) {
     std::snprintf(buf, buflen, "%dx%d,%s,%s,%s", s.nt, s.cpl,
                   placement_name(s.im),
                   s.sym ? "sym" : "general", d.sparse_upd ? "sparse-flow" : "dense-flow");
+    return QAP_PT_OK;
+}
+
+int qap_pt_pool_bucket(qap_pt_pool *p, int32_t b, char *shape, int32_t buflen,
+                       int64_t *replicas, double *kernel_ms, int64_t *launches) {
+    if (!p) return fail(QAP_PT_E_INVAL, "pool is NULL");
+    if (b < 0 || b >= (int32_t)p->buckets.size())
+        return fail(QAP_PT_E_INVAL, "bucket %d out of range (%d buckets)", b,
+                    (int)p->buckets.size());
+    const Bucket &B = p->buckets[b];
+    if (shape && buflen > 0)
+        std::snprintf(shape, buflen, "%dx%d,%s,%s%s", B.shape.nt, B.shape.cpl,
+                      placement_name(B.shape.im), B.shape.sym ? "sym" : "general",
+                      B.shape.sparse ? ",sparse-flow" : "");
+    if (replicas) *replicas = (int64_t)B.reps.size();
+    if (kernel_ms) *kernel_ms = b < (int32_t)p->bucket_ms.size() ? p->bucket_ms[b] : 0.0;
+    if (launches) *launches = b < (int32_t)p->bucket_launches.size() ? p->bucket_launches[b] : 0;
     return QAP_PT_OK;
 }
 

@@ -59,9 +59,17 @@ struct qap_pt_pool {
     std::vector<hipStream_t> side;     // one per extra bucket (default priority)
     hipEvent_t ev_fork = nullptr;
     std::vector<hipEvent_t> ev_join;
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    int graph_depth = 0;
+    // captured exchange intervals by depth: a run replays the deepest graph that fits and
+    // then one graph of the remainder, so no replay carries idle intervals
+    struct CapturedRun {
+        int depth;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<CapturedRun> graphs;
+    // timed runs: summed device time (ms) and count of each bucket's sweep launches
+    std::vector<double> bucket_ms;
+    std::vector<int64_t> bucket_launches;
     double seconds = 0.0;
     // slot mode: `count` fixed-capacity slots that instances are loaded into and released
     // from while the pool runs (qap_pt_slots.cpp)
