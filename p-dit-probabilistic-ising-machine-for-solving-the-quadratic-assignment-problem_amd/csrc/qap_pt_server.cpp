@@ -13,6 +13,12 @@
 // A request's trajectory depends only on its own inputs and seed (Philox counters are
 // keyed by the request's seed and replica index, never by slot or bank), so its result
 // is the same alone and in any mix.  Banks idle for a second are freed.
+//
+// Locking: the server mutex guards only what client threads touch -- the incoming queue,
+// the per-ticket records clients poll, and the published counters.  The worker takes new
+// requests in one short critical section, does every upload, launch and result copy with
+// the lock released, and publishes finished results in another; a request's clock starts
+// when qap_srv_submit is entered, before any lock.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -67,26 +73,44 @@ constexpr double kBankIdleFree = 1.0;    // seconds an empty bank is kept
 
 }  // namespace
 
+// What a client sees of a request: its status and, once final, its result.
+struct Record {
+    int32_t status = QAP_SRV_QUEUED;
+    qap_pt_result res{};
+    std::vector<int32_t> best_perm;
+};
+
 struct qap_srv {
     qap_srv_config cfg{};
+    // ---- shared with client threads, guarded by mu
     std::mutex mu;
-    std::condition_variable cv;
-    std::deque<uint64_t> pending;
-    std::unordered_map<uint64_t, Request> reqs;
-    std::vector<Flight> flights;
-    std::vector<Bank> banks;
+    std::condition_variable cv;       // the worker waits here for requests
+    std::condition_variable done_cv;  // waiters wait here for results
+    std::deque<std::pair<uint64_t, Request>> incoming;  // submitted, not yet taken
+    std::unordered_map<uint64_t, Record> recs;
     uint64_t next_ticket = 1;
-    int64_t bytes_in_use = 0;
     int64_t n_done = 0;
+    int64_t pub_queued = 0, pub_running = 0, pub_bytes = 0;  // the worker's counts
     bool stop = false;
     std::thread worker;
+    // ---- the worker's own (no lock)
+    std::deque<uint64_t> queue;                     // taken, waiting for admission
+    std::unordered_map<uint64_t, Request> work;     // bodies of queued and running requests
+    std::vector<uint64_t> outbox;                   // final since the last publish
+    std::vector<uint64_t> started_run;              // became RUNNING since the last publish
+    std::vector<Flight> flights;
+    std::vector<Bank> banks;
+    int64_t bytes_in_use = 0;
+    bool stopping = false;
 };
 
 namespace {
 
 qap_problem problem_of(const Request &r) { return qap_problem{r.n, r.F.data(), r.D.data()}; }
 
-// Finished pools: copy results out, release the pool.  Called with the lock held.
+void finish(qap_srv *s, uint64_t t, int status);
+
+// Finished pools: copy results out, release the pool.  Worker only, lock not held.
 void collect(qap_srv *s) {
     for (size_t f = 0; f < s->flights.size();) {
         Flight &fl = s->flights[f];
@@ -95,17 +119,14 @@ void collect(qap_srv *s) {
             ++f;
             continue;
         }
-        const auto now = Clock::now();
         for (size_t k = 0; k < fl.tickets.size(); ++k) {
-            Request &r = s->reqs[fl.tickets[k]];
+            Request &r = s->work[fl.tickets[k]];
             r.res = qap_pt_result{};
             r.res.best_perm = r.best_perm.data();
             int rc = (q == hipSuccess) ? qap::pool_result_internal(fl.pool, (int)k, &r.res)
                                        : QAP_PT_E_DEVICE;
             r.res.best_perm = nullptr;
-            r.res.seconds = std::chrono::duration<double>(now - r.submitted).count();
-            r.status = rc ? rc : QAP_SRV_DONE;
-            ++s->n_done;
+            finish(s, fl.tickets[k], rc ? rc : QAP_SRV_DONE);
         }
         (void)hipEventDestroy(fl.done);
         qap::pool_destroy_internal(fl.pool);
@@ -121,10 +142,12 @@ int64_t available_bytes(qap_srv *s) {
     return (int64_t)free_b - s->cfg.reserve_bytes;
 }
 
-void finish(qap_srv *s, Request &r, int status) {
+// A request's final status; its result goes out with the next publish.  Worker only.
+void finish(qap_srv *s, uint64_t t, int status) {
+    Request &r = s->work[t];
     r.status = status;
     r.res.seconds = std::chrono::duration<double>(Clock::now() - r.submitted).count();
-    ++s->n_done;
+    s->outbox.push_back(t);
 }
 
 // Bank class of a request: the smallest size class that holds n, and 32 or 64 replicas;
@@ -143,7 +166,7 @@ void bank_class(const Request &r, int &n_cap, int &rt_cap) {
 }
 
 // Launches that ended: finished requests leave their slots with their results; requests
-// past their timeout are halted (they finish after the next launch).  Lock held.
+// past their timeout are halted (they finish after the next launch).  Worker only.
 void service_banks(qap_srv *s) {
     const auto now = Clock::now();
     std::vector<int> done;
@@ -164,19 +187,19 @@ void service_banks(qap_srv *s) {
         for (int slot = 0; slot < (int)b.ticket.size() && rc; ++slot)
             if (b.ticket[slot]) done.push_back(slot);  // device error: fail every request
         for (int slot : done) {
-            Request &r = s->reqs[b.ticket[slot]];
+            Request &r = s->work[b.ticket[slot]];
             r.res = qap_pt_result{};
             r.res.best_perm = r.best_perm.data();
             const int rr = rc ? rc : qap::pool_result_internal(b.pool, slot, &r.res);
             r.res.best_perm = nullptr;
-            finish(s, r, rr ? rr : QAP_SRV_DONE);
+            finish(s, b.ticket[slot], rr ? rr : QAP_SRV_DONE);
             (void)qap::slots_release(b.pool, slot);
             b.ticket[slot] = 0;
             b.halting[slot] = 0;
         }
         for (int slot = 0; slot < (int)b.ticket.size() && !rc; ++slot) {
             if (!b.ticket[slot] || b.halting[slot]) continue;
-            const Request &r = s->reqs[b.ticket[slot]];
+            const Request &r = s->work[b.ticket[slot]];
             if (r.par.timeout_s > 0 &&
                 std::chrono::duration<double>(now - b.started[slot]).count() > r.par.timeout_s) {
                 (void)qap::slots_halt(b.pool, slot);
@@ -195,7 +218,7 @@ void service_banks(qap_srv *s) {
     for (size_t k = 0; k < s->banks.size();) {
         Bank &b = s->banks[k];
         if (!b.inflight && b.idle_since != Clock::time_point{} &&
-            (s->stop || std::chrono::duration<double>(now - b.idle_since).count() > kBankIdleFree)) {
+            (s->stopping || std::chrono::duration<double>(now - b.idle_since).count() > kBankIdleFree)) {
             (void)hipEventDestroy(b.done);
             qap::pool_destroy_internal(b.pool);
             s->bytes_in_use -= b.bytes;
@@ -207,7 +230,7 @@ void service_banks(qap_srv *s) {
 }
 
 // A bank with a free slot for this class that is between launches, creating one when
-// the budget holds it.  nullptr: wait (or use a pool of its own).  Lock held.
+// the budget holds it.  nullptr: wait (or use a pool of its own).  Worker only.
 Bank *bank_for(qap_srv *s, int n_cap, int rt_cap, int &err) {
     err = QAP_PT_OK;
     bool busy_room = false;
@@ -246,9 +269,9 @@ Bank *bank_for(qap_srv *s, int n_cap, int rt_cap, int &err) {
     return &s->banks.back();
 }
 
-// A request in a pool of its own (sizes no bank class holds).  Lock held.
+// A request in a pool of its own (sizes no bank class holds).  Worker only.
 int start_flight(qap_srv *s, uint64_t t) {
-    Request &r = s->reqs[t];
+    Request &r = s->work[t];
     Flight fl;
     fl.tickets = {t};
     fl.bytes = r.bytes;
@@ -263,25 +286,26 @@ int start_flight(qap_srv *s, uint64_t t) {
         return rc;
     }
     r.status = QAP_SRV_RUNNING;
+    s->started_run.push_back(t);
     s->bytes_in_use += fl.bytes;
     s->flights.push_back(fl);
     return QAP_PT_OK;
 }
 
 // Admit queued requests in arrival order while slots and memory allow; a request that
-// has to wait does not hold back later ones of other classes.  Lock held.
+// has to wait does not hold back later ones of other classes.  Worker only.
 void admit(qap_srv *s) {
-    if (s->pending.empty()) return;
-    if (s->cfg.batch_window_s > 0 && (int)s->pending.size() < s->cfg.max_batch) {
-        const Request &oldest = s->reqs[s->pending.front()];
+    if (s->queue.empty()) return;
+    if (s->cfg.batch_window_s > 0 && (int)s->queue.size() < s->cfg.max_batch) {
+        const Request &oldest = s->work[s->queue.front()];
         if (std::chrono::duration<double>(Clock::now() - oldest.submitted).count() <
             s->cfg.batch_window_s)
             return;
     }
     const auto now = Clock::now();
-    for (auto it = s->pending.begin(); it != s->pending.end();) {
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
         const uint64_t t = *it;
-        Request &r = s->reqs[t];
+        Request &r = s->work[t];
         int n_cap = 0, rt_cap = 0, err = QAP_PT_OK;
         bank_class(r, n_cap, rt_cap);
         Bank *b = n_cap ? bank_for(s, n_cap, rt_cap, err) : nullptr;
@@ -290,18 +314,19 @@ void admit(qap_srv *s) {
             const qap_problem pb = problem_of(r);
             const int rc = qap::slots_load(b->pool, slot, pb, r.par);
             if (rc) {
-                finish(s, r, rc);
+                finish(s, t, rc);
             } else {
                 b->ticket[slot] = t;
                 b->started[slot] = now;
                 r.status = QAP_SRV_RUNNING;
+                s->started_run.push_back(t);
             }
-            it = s->pending.erase(it);
+            it = s->queue.erase(it);
             continue;
         }
         if (err != QAP_PT_OK && err != QAP_PT_E_NOMEM) {  // the device refused a bank
-            finish(s, r, err);
-            it = s->pending.erase(it);
+            finish(s, t, err);
+            it = s->queue.erase(it);
             continue;
         }
         // no bank room: a pool of its own when the budget holds it (sizes no bank class
@@ -311,20 +336,20 @@ void admit(qap_srv *s) {
         if (alone && (int)(s->banks.size() + s->flights.size()) < s->cfg.max_pools &&
             r.bytes <= available_bytes(s)) {
             const int rc = start_flight(s, t);
-            if (rc) finish(s, r, rc);
-            it = s->pending.erase(it);
+            if (rc) finish(s, t, rc);
+            it = s->queue.erase(it);
             continue;
         }
         if (alone && s->banks.empty() && s->flights.empty()) {  // nothing will free memory
-            finish(s, r, QAP_PT_E_NOMEM);
-            it = s->pending.erase(it);
+            finish(s, t, QAP_PT_E_NOMEM);
+            it = s->queue.erase(it);
             continue;
         }
         ++it;
     }
 }
 
-// Banks between launches with loaded requests start their next launch.  Lock held.
+// Banks between launches with loaded requests start their next launch.  Worker only.
 void launch_banks(qap_srv *s) {
     for (Bank &b : s->banks) {
         if (b.inflight || qap::slots_in_use(b.pool) == 0) continue;
@@ -332,7 +357,7 @@ void launch_banks(qap_srv *s) {
         if (rc) {  // fail the bank's requests
             for (int slot = 0; slot < (int)b.ticket.size(); ++slot)
                 if (b.ticket[slot]) {
-                    finish(s, s->reqs[b.ticket[slot]], rc);
+                    finish(s, b.ticket[slot], rc);
                     (void)qap::slots_release(b.pool, slot);
                     b.ticket[slot] = 0;
                 }
@@ -343,23 +368,67 @@ void launch_banks(qap_srv *s) {
     }
 }
 
+// Hands finished results and status changes to the clients, with the worker's counts.
+// Lock held.
+void publish_locked(qap_srv *s) {
+    for (uint64_t t : s->started_run) {
+        auto it = s->recs.find(t);
+        if (it != s->recs.end() && it->second.status == QAP_SRV_QUEUED)
+            it->second.status = QAP_SRV_RUNNING;
+    }
+    s->started_run.clear();
+    for (uint64_t t : s->outbox) {
+        Request &r = s->work[t];
+        Record &rec = s->recs[t];
+        rec.status = r.status;
+        rec.res = r.res;
+        rec.best_perm = std::move(r.best_perm);
+        s->work.erase(t);
+        ++s->n_done;
+    }
+    s->outbox.clear();
+    int64_t run = 0;
+    for (auto &f : s->flights) run += (int64_t)f.tickets.size();
+    for (auto &b : s->banks) run += qap::slots_in_use(b.pool);
+    s->pub_queued = (int64_t)s->queue.size();
+    s->pub_running = run;
+    s->pub_bytes = s->bytes_in_use;
+}
+
 void worker_main(qap_srv *s) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
-        if (s->stop) {
-            for (uint64_t t : s->pending) finish(s, s->reqs[t], QAP_PT_E_BUSY);
-            s->pending.clear();
+        {  // take what was submitted
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (auto &in : s->incoming) {
+                s->queue.push_back(in.first);
+                s->work.emplace(in.first, std::move(in.second));
+            }
+            s->incoming.clear();
+            s->stopping = s->stop;
         }
+        if (s->stopping) {
+            for (uint64_t t : s->queue) finish(s, t, QAP_PT_E_BUSY);
+            s->queue.clear();
+        }
+        // device work, lock released
         collect(s);
         service_banks(s);
-        if (s->stop && s->flights.empty() && s->banks.empty()) break;
-        admit(s);
-        launch_banks(s);
+        const bool exit = s->stopping && s->flights.empty() && s->banks.empty();
+        if (!exit) {
+            admit(s);
+            launch_banks(s);
+        }
         bool busy = !s->flights.empty();
         for (const Bank &b : s->banks) busy = busy || b.inflight;
-        s->cv.wait_for(lk, std::chrono::microseconds(busy ? 100 : 2000));
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            const bool any_done = !s->outbox.empty();
+            publish_locked(s);
+            if (any_done) s->done_cv.notify_all();
+            if (exit) break;
+            if (s->incoming.empty() && !s->stop)
+                s->cv.wait_for(lk, std::chrono::microseconds(busy ? 100 : 2000));
+        }
     }
 }
 
@@ -390,6 +459,7 @@ int qap_srv_create(const qap_srv_config *cfg, qap_srv **out) {
 }
 
 int qap_srv_submit(qap_srv *s, const qap_problem *prob, const qap_pt_params *par, uint64_t *ticket) {
+    const auto submitted = Clock::now();  // the request's clock starts before any lock
     if (!s || !prob || !par || !ticket) return qap::fail(QAP_PT_E_INVAL, "NULL argument");
     if (prob->n < 1 || prob->n > QAP_PT_MAX_N || !prob->F || !prob->D || !par->beta_ln ||
         par->R < 1 || par->T < 1)
@@ -402,27 +472,23 @@ int qap_srv_submit(qap_srv *s, const qap_problem *prob, const qap_pt_params *par
         r.D.assign(prob->D, prob->D + nn);
         r.betas.assign(par->beta_ln, par->beta_ln + par->R);
         r.par = *par;
-        r.par.beta_ln = nullptr;
         if (par->init_perm) r.init.assign(par->init_perm, par->init_perm + prob->n);
-        r.par.init_perm = nullptr;
         r.best_perm.assign(prob->n, 0);
+        // pointers into the request's own storage (the vectors keep their buffers when
+        // the request is moved)
+        r.par.beta_ln = r.betas.data();
+        r.par.init_perm = r.init.empty() ? nullptr : r.init.data();
         qap_problem pb{r.n, r.F.data(), r.D.data()};
-        qap_pt_params pp = r.par;
-        pp.beta_ln = r.betas.data();
-        r.bytes = qap::pool_device_bytes(1, &pb, &pp);
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->stop) return qap::fail(QAP_PT_E_BUSY, "server is shutting down");
+        r.bytes = qap::pool_device_bytes(1, &pb, &r.par);
         if (s->cfg.device_budget > 0 && r.bytes > s->cfg.device_budget)
             return qap::fail(QAP_PT_E_NOMEM, "request needs %lld device bytes, budget is %lld",
                              (long long)r.bytes, (long long)s->cfg.device_budget);
-        r.submitted = Clock::now();
+        r.submitted = submitted;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return qap::fail(QAP_PT_E_BUSY, "server is shutting down");
         const uint64_t t = s->next_ticket++;
-        auto &slot = s->reqs[t];
-        slot = std::move(r);
-        // pointers into the request's own storage (stable: the vectors are not resized)
-        slot.par.beta_ln = slot.betas.data();
-        slot.par.init_perm = slot.init.empty() ? nullptr : slot.init.data();
-        s->pending.push_back(t);
+        s->recs.emplace(t, Record{});
+        s->incoming.emplace_back(t, std::move(r));
         *ticket = t;
     } catch (...) {
         return qap::fail(QAP_PT_E_NOMEM, "host allocation failed");
@@ -432,10 +498,10 @@ int qap_srv_submit(qap_srv *s, const qap_problem *prob, const qap_pt_params *par
 }
 
 static int poll_locked(qap_srv *s, uint64_t ticket, qap_pt_result *res, int32_t *status) {
-    auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end()) return qap::fail(QAP_PT_E_INVAL, "unknown ticket %llu",
+    auto it = s->recs.find(ticket);
+    if (it == s->recs.end()) return qap::fail(QAP_PT_E_INVAL, "unknown ticket %llu",
                                               (unsigned long long)ticket);
-    Request &r = it->second;
+    Record &r = it->second;
     *status = r.status;
     if (r.status == QAP_SRV_DONE || r.status < 0) {
         if (res) {
@@ -445,7 +511,7 @@ static int poll_locked(qap_srv *s, uint64_t ticket, qap_pt_result *res, int32_t 
             if (dst && r.status == QAP_SRV_DONE)
                 std::copy(r.best_perm.begin(), r.best_perm.end(), dst);
         }
-        s->reqs.erase(it);
+        s->recs.erase(it);
     }
     return QAP_PT_OK;
 }
@@ -460,21 +526,21 @@ int qap_srv_wait(qap_srv *s, uint64_t ticket, double timeout_s, qap_pt_result *r
                  int32_t *status) {
     if (!s || !status) return qap::fail(QAP_PT_E_INVAL, "NULL argument");
     const auto t0 = Clock::now();
+    std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            auto it = s->reqs.find(ticket);
-            if (it == s->reqs.end())
-                return qap::fail(QAP_PT_E_INVAL, "unknown ticket %llu", (unsigned long long)ticket);
-            if (it->second.status == QAP_SRV_DONE || it->second.status < 0)
-                return poll_locked(s, ticket, res, status);
-            if (timeout_s > 0 &&
-                std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s) {
-                *status = it->second.status;
-                return QAP_PT_OK;
-            }
+        auto it = s->recs.find(ticket);
+        if (it == s->recs.end())
+            return qap::fail(QAP_PT_E_INVAL, "unknown ticket %llu", (unsigned long long)ticket);
+        if (it->second.status == QAP_SRV_DONE || it->second.status < 0)
+            return poll_locked(s, ticket, res, status);
+        const double left =
+            timeout_s > 0 ? timeout_s - std::chrono::duration<double>(Clock::now() - t0).count()
+                          : 0.05;
+        if (timeout_s > 0 && left <= 0) {
+            *status = it->second.status;
+            return QAP_PT_OK;
         }
-        std::this_thread::sleep_for(std::chrono::microseconds(200));
+        s->done_cv.wait_for(lk, std::chrono::duration<double>(std::min(left, 0.05)));
     }
 }
 
@@ -482,13 +548,10 @@ int qap_srv_stats(qap_srv *s, int64_t *queued, int64_t *running, int64_t *done,
                   int64_t *device_bytes) {
     if (!s) return qap::fail(QAP_PT_E_INVAL, "NULL argument");
     std::lock_guard<std::mutex> lk(s->mu);
-    int64_t run = 0;
-    for (auto &f : s->flights) run += (int64_t)f.tickets.size();
-    for (auto &b : s->banks) run += qap::slots_in_use(b.pool);
-    if (queued) *queued = (int64_t)s->pending.size();
-    if (running) *running = run;
+    if (queued) *queued = (int64_t)s->incoming.size() + s->pub_queued;
+    if (running) *running = s->pub_running;
     if (done) *done = s->n_done;
-    if (device_bytes) *device_bytes = s->bytes_in_use;
+    if (device_bytes) *device_bytes = s->pub_bytes;
     return QAP_PT_OK;
 }
 

@@ -111,3 +111,34 @@ def test_one_shot_requests_run_beside_sweep_requests(qpt, oracle, gpu):
         assert got[k]["best_cost"] == ref["best_cost"], k
         np.testing.assert_array_equal(got[k]["best_perm"], ref["best_perm"])
         assert got[k]["accepted"] == ref["accepted"]
+
+
+def test_submit_returns_while_large_requests_are_admitted(qpt, gpu):
+    """Three n = 1024 requests (64 replicas each: host preparation, uploads, force-matrix
+    builds of about 0.1 s each) are being admitted while small requests arrive.  Submit
+    only copies the request and queues it under a short lock -- the worker does device
+    work with the lock released -- so every submit returns within milliseconds, and the
+    latencies count from the moment submit was called."""
+    F, D = qpt.instances.large_c5(1024)
+    big = [qpt.Instance(F=F, D=D, beta_ln=qpt.instances.geometric_ladder(64, 300.0, 30000.0),
+                        sweeps=1, seed=40 + k) for k in range(3)]
+    small = _requests(qpt, 12, seed0=950)
+    alone = [qpt.solve(r) for r in small]
+    with qpt.Server(max_pools=8, max_batch=4) as srv:
+        tb = [srv.submit(r) for r in big]
+        time.sleep(0.01)  # the worker is now preparing the first large request
+        took, ts = [], []
+        for r in small:
+            t0 = time.perf_counter()
+            ts.append(srv.submit(r))
+            took.append(time.perf_counter() - t0)
+        got = [srv.wait(t, timeout_s=120.0) for t in ts]
+        gb = [srv.wait(t, timeout_s=120.0) for t in tb]
+    print("submit seconds:", [round(x, 4) for x in took], flush=True)
+    assert max(took) < 0.02, took
+    for k in range(len(small)):
+        assert got[k]["status"] == qpt.QAP_SRV_DONE
+        assert got[k]["best_cost"] == alone[k]["best_cost"]
+        assert got[k]["seconds"] > 0
+    for g in gb:
+        assert g["status"] == qpt.QAP_SRV_DONE
