@@ -110,15 +110,15 @@ namespace qap {
 int hbm_stride(int n) { return round4i(n + 1); }
 
 int device_cus() {
-    static int cus = 0;
-    if (cus == 0) {
+    // read once per process (thread-safe initialisation: pools may be created on several
+    // host threads)
+    static const int cus = [] {
         int dev = 0, v = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
-            cus = v;
-        else
-            cus = 256;  // MI355X, for shape reports without a device
-    }
+            return v;
+        return 256;  // MI355X, for shape reports without a device
+    }();
     return cus;
 }
 
