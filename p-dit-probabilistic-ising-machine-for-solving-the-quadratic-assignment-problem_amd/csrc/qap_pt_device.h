@@ -130,6 +130,9 @@ struct PoolDev {
     int32_t *order;           // per ladder: position -> replica (local index)
     int32_t *tss;             // per ladder: timeSinceSwap
     int64_t *ladder_best;     // per ladder
+    uint32_t *sync;           // persistent sweep launches: [0] fault word (a wait that gave
+                              //   up), then per instance [1 + 2 k] arrivals and [2 + 2 k] the
+                              //   number of exchange steps run inside persistent launches
 };
 
 }  // namespace qap

@@ -66,6 +66,14 @@ bool oneshot_fits(int n);
 
 hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,
                         int lds_bytes, hipStream_t s);
+// `intervals` exchange intervals of an LDS-placement bucket in one cooperative launch: the
+// force matrices stay in LDS and the exchange steps between intervals run in the kernel
+// (the first interval's step must have run before the launch).  Needs PoolDev::sync;
+// hipErrorCooperativeLaunchTooLarge when the workgroups cannot all be resident.
+hipError_t launch_sweep_persistent(const PoolDev &P, const SweepShape &sh, const int32_t *list,
+                                   int nblocks, int lds_bytes, int intervals, hipStream_t s);
+// whether a persistent launch of nblocks workgroups of this shape fits the device at once
+bool sweep_persistent_fits(const SweepShape &sh, int nblocks, int lds_bytes);
 hipError_t launch_sweep_lazy(const PoolDev &P, const SweepShape &sh, const int32_t *list,
                              int nblocks, int lds_bytes, hipStream_t s);
 hipError_t launch_oneshot(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,

@@ -324,7 +324,7 @@ void pool_free(qap_pt_pool *p) {
                     p->d_Ft,    p->d_Dt,        p->d_Fdiag,    p->d_Ddiag,  p->d_beta2f,
                     p->d_c20,   p->d_perm,      p->d_pinv,     p->d_best_perm, p->d_I,
                     p->d_E,     p->d_bestE,     p->d_limits,   p->d_accepted, p->d_cells, p->d_pos,
-                    p->d_order, p->d_tss,       p->d_ladder_best};
+                    p->d_order, p->d_tss,       p->d_ladder_best, p->d_sync};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto st : p->side) (void)hipStreamDestroy(st);
@@ -421,7 +421,36 @@ int graph_for(qap_pt_pool *p, int depth, hipGraphExec_t *out) {
 // (up to kGraphDepth intervals) and one of the remainder, or, for an arena pool, the same
 // launches one by one.
 constexpr int kGraphDepth = 32;
+// exchange intervals per persistent launch at most (bounds one launch's duration)
+constexpr int kPersistMaxIntervals = 4096;
+
+// One persistent launch per bucket for `intervals` intervals, after the exchange step of
+// the first one.  Returns hipErrorCooperativeLaunchTooLarge (nothing launched) when a
+// bucket's workgroups cannot all be resident.
+hipError_t enqueue_persistent(qap_pt_pool *p, int intervals, const hipEvent_t *ev) {
+    hipError_t e = launch_exchange(p->P, p->count, p->max_rt, p->stream);
+    for (size_t b = 0; b < p->buckets.size() && e == hipSuccess; ++b) {
+        const Bucket &B = p->buckets[b];
+        if (ev) e = hipEventRecord(ev[2 * b], p->stream);
+        if (e == hipSuccess)
+            e = launch_sweep_persistent(p->P, B.shape, B.d_list, (int)B.reps.size(), B.lds_bytes,
+                                        intervals, p->stream);
+        if (e == hipSuccess && ev) e = hipEventRecord(ev[2 * b + 1], p->stream);
+    }
+    return e;
+}
+
 int enqueue_intervals(qap_pt_pool *p, int64_t intervals) {
+    if (p->persistent) {
+        for (int64_t m = 0; m < intervals; m += kPersistMaxIntervals) {
+            const int n = (int)std::min<int64_t>(intervals - m, kPersistMaxIntervals);
+            const hipError_t e = enqueue_persistent(p, n, nullptr);
+            if (e != hipSuccess)
+                return fail(QAP_PT_E_DEVICE, "persistent sweep launch failed: %s",
+                            hipGetErrorString(e));
+        }
+        return QAP_PT_OK;
+    }
     if (p->plain_launch) {
         // no side streams, events or graph objects: every bucket in turn on the pool stream
         for (int64_t x = 0; x < intervals; ++x) {
@@ -592,6 +621,7 @@ void bind_pool(qap_pt_pool *p) {
     P.order = p->d_order;
     P.tss = p->d_tss;
     P.ladder_best = p->d_ladder_best;
+    P.sync = p->d_sync;
 }
 
 }  // namespace qap
@@ -697,6 +727,16 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     }
 
     for (auto &B : p->buckets) order_bucket(p, B, [&](int32_t g) { return rep_inst[g]; });
+    // Persistent launches: a latency-bound pool (a CU per replica) whose force matrices all
+    // sit in LDS keeps them there across exchange intervals, the exchange steps running in
+    // the kernel; QAP_PT_PERSIST=0 runs every interval as its own launches instead.
+    {
+        const char *env = std::getenv("QAP_PT_PERSIST");
+        p->persistent = !arena && !throughput && !(env && env[0] == '0') &&
+                        p->buckets.size() == 1 &&
+                        sweep_persistent_fits(p->buckets[0].shape, (int)p->buckets[0].reps.size(),
+                                              p->buckets[0].lds_bytes);
+    }
 
     auto bail = [&](int rc) {
         pool_free(p);
@@ -738,6 +778,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(palloc(p, &p->d_order, RTall));
     POOL_TRY(palloc(p, &p->d_tss, p->total_ladders));
     POOL_TRY(palloc(p, &p->d_ladder_best, p->total_ladders));
+    POOL_TRY(palloc(p, &p->d_sync, 1 + 2 * (size_t)count));
     for (auto &b : p->buckets) POOL_TRY(palloc(p, &b.d_list, b.reps.size()));
     if (p->arena && p->arena_used > p->arena_size)
         return bail(fail(QAP_PT_E_NOMEM,
@@ -784,6 +825,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(up(p->d_ladder_best, ladder_best.data(), 8 * ladder_best.size()));
     POOL_TRY(hipMemsetAsync(p->d_I, 0, sizeof(int32_t) * std::max<int64_t>(p->total_I, 1),
                             p->stream));
+    POOL_TRY(hipMemsetAsync(p->d_sync, 0, sizeof(uint32_t) * (1 + 2 * (size_t)count), p->stream));
 
     bind_pool(p);
 
@@ -845,8 +887,10 @@ int pool_run_impl(qap_pt_pool *p, int64_t sweeps, double *kernel_ms, int64_t *la
         const int nb = (int)p->buckets.size();
         p->bucket_ms.resize(nb, 0.0);
         p->bucket_launches.resize(nb, 0);
-        constexpr int chunk = 64;  // intervals between host reads of the events
-        std::vector<hipEvent_t> ev(2 * chunk * std::max(nb, 1), nullptr);
+        // persistent pools: one launch per bucket covers up to kPersistMaxIntervals
+        // intervals, bracketed by events on the pool stream
+        const int chunk = p->persistent ? kPersistMaxIntervals : 64;
+        std::vector<hipEvent_t> ev(2 * (p->persistent ? 1 : chunk) * std::max(nb, 1), nullptr);
         int rc = QAP_PT_OK;
         for (auto &e : ev)
             if (hipEventCreate(&e) != hipSuccess) rc = fail(QAP_PT_E_DEVICE, "hipEventCreate failed");
@@ -855,7 +899,18 @@ int pool_run_impl(qap_pt_pool *p, int64_t sweeps, double *kernel_ms, int64_t *la
         int64_t m = 0;
         while (m < max_new && rc == QAP_PT_OK) {
             int used = 0;
-            for (; used < chunk && m < max_new; ++used, ++m) {
+            if (p->persistent) {
+                const int n = (int)std::min<int64_t>(max_new - m, chunk);
+                const hipError_t e = enqueue_persistent(p, n, ev.data());
+                if (e != hipSuccess) {
+                    rc = fail(QAP_PT_E_DEVICE, "persistent sweep launch failed: %s",
+                              hipGetErrorString(e));
+                    break;
+                }
+                used = 1;
+                m += n;
+            }
+            for (; !p->persistent && used < chunk && m < max_new; ++used, ++m) {
                 const hipError_t e = enqueue_interval(p, ev.data() + 2 * used * nb);
                 if (e != hipSuccess) {
                     rc = fail(QAP_PT_E_DEVICE, "interval launch failed: %s", hipGetErrorString(e));
@@ -912,6 +967,13 @@ int pool_run_impl(qap_pt_pool *p, int64_t sweeps, double *kernel_ms, int64_t *la
     }
     HIP_TRY(hipStreamSynchronize(p->stream));
     p->seconds += elapsed();
+    if (p->persistent) {
+        uint32_t fault = 0;
+        HIP_TRY(hipMemcpy(&fault, p->d_sync, sizeof(fault), hipMemcpyDeviceToHost));
+        if (fault)
+            return fail(QAP_PT_E_DEVICE, "a persistent sweep launch gave up waiting for the "
+                                         "replicas of its ladder");
+    }
     return QAP_PT_OK;
 }
 

@@ -55,6 +55,10 @@ struct qap_pt_pool {
     int64_t *d_E = nullptr, *d_bestE = nullptr, *d_limits = nullptr;
     uint64_t *d_accepted = nullptr, *d_cells = nullptr;
     int32_t *d_pos = nullptr, *d_order = nullptr, *d_tss = nullptr;
+    uint32_t *d_sync = nullptr;  // persistent launches: fault word + per instance 2 words
+    // latency-bound pool of LDS placements: exchange intervals run as persistent
+    // cooperative launches (replicas stay in LDS, exchange steps in the kernel)
+    bool persistent = false;
     int64_t *d_ladder_best = nullptr;
     std::vector<hipStream_t> side;     // one per extra bucket (default priority)
     hipEvent_t ev_fork = nullptr;

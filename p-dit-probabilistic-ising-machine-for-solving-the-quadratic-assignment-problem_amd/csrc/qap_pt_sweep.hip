@@ -219,8 +219,23 @@ struct IElem<IM_LDS16> {
     using T = uint16_t;  // holds -I
 };
 
+__device__ void exchange_instance(const PoolDev &P, int inst, int cap, unsigned char *ex_smem,
+                                  int tid);
+
+// Longest a persistent workgroup waits for the rest of its ladder at an exchange step:
+// 30 s of the 100 MHz real-time counter.  The launch is cooperative (every workgroup
+// resident), so the wait ends when the slowest replica finishes its interval; the limit
+// only turns a fault into a reported error (the fault word) instead of a hang.
+constexpr uint64_t kPersistWaitTicks = 3000000000ull;
+
+// max_iv > 1 (persistent launch, LDS placements): the workgroup runs up to max_iv exchange
+// intervals with its replica resident in LDS.  Between intervals it publishes its cost and
+// best and arrives at its instance's counter; the last replica of the instance to arrive
+// runs the instance's exchange step (the one exchange_kernel runs between launches) and
+// releases the others by advancing the instance's generation word.
 template <int NT, int CPL, int IM, bool SYM, bool SP>
-__global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__restrict__ rep_list) {
+__global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__restrict__ rep_list,
+                                                   int max_iv) {
     constexpr int NW = NT / kWave;
     using IT = typename IElem<IM>::T;
     extern __shared__ __align__(16) int32_t smem[];
@@ -344,10 +359,15 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     bool improved = false;  // bestp_s holds this launch's best permutation
     uint64_t acc = P.accepted[g];
     uint64_t cells = 0;  // this wave's share: n x the nonzero flow differences it owns
-    const int pos = P.pos[g];
-    const float beta2f = P.beta2f[d.beta_off + pos];
-    const double c20 = P.c20[d.beta_off + pos];
-    const uint32_t sweep0 = P.state[inst].sweep_base;
+    int pos = P.pos[g];
+    float beta2f = P.beta2f[d.beta_off + pos];
+    double c20 = P.c20[d.beta_off + pos];
+    uint32_t sweep0 = P.state[inst].sweep_base;
+    // exchange steps of this instance done inside persistent launches so far (read before
+    // this workgroup first arrives, so no step of this launch can have advanced it)
+    uint32_t gen0 = 0;
+    if (max_iv > 1)
+        gen0 = __hip_atomic_load(&P.sync[2 + 2 * inst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // SP: the launch holds sparse-flow instances only, and the dense update path is not
     // built (grey n = 256: 2.2 % faster; the sparse path is exact for any flow)
     const bool sparse_upd = SP || d.sparse_upd != 0;
@@ -366,6 +386,54 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     // swap at site i moves it only when the partner location is the next site)
     int a_reg = a_cur;
 
+    for (int iv = 0; iv < max_iv; ++iv) {
+    if (iv > 0) {
+        // ---- exchange step between intervals (persistent launch) ----------------
+        if (tid == 0) {  // this replica's cost and best, for the exchange step
+            P.E[g] = E;
+            P.bestE[g] = bestE;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            const uint32_t prev = __hip_atomic_fetch_add(&P.sync[1 + 2 * inst], 1u, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            red_s[100] = prev == (uint32_t)(d.R * d.T - 1);  // the instance's last arrival
+            red_s[101] = 0;
+        }
+        __syncthreads();
+        const uint32_t target = gen0 + (uint32_t)iv;
+        if (red_s[100]) {
+            if (wid == 0) {  // every other replica has arrived: run the step, release them
+                if (lane == 0)
+                    __hip_atomic_store(&P.sync[1 + 2 * inst], 0u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                exchange_instance(P, inst, 0, nullptr, lane);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                if (lane == 0)
+                    __hip_atomic_store(&P.sync[2 + 2 * inst], target, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (tid == 0) {
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            while (__hip_atomic_load(&P.sync[2 + 2 * inst], __ATOMIC_ACQUIRE,
+                                     __HIP_MEMORY_SCOPE_AGENT) != target) {
+                __builtin_amdgcn_s_sleep(2);
+                if (__builtin_amdgcn_s_memrealtime() - t0 > kPersistWaitTicks) {
+                    __hip_atomic_store(&P.sync[0], 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    red_s[101] = 1;
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (red_s[101]) break;  // gave up waiting: the host reports the fault
+        const InstState stn = P.state[inst];
+        if (!stn.active) break;  // limit or goal reached at this step
+        pos = P.pos[g];
+        beta2f = P.beta2f[d.beta_off + pos];
+        c20 = P.c20[d.beta_off + pos];
+        sweep0 = stn.sweep_base;
+    }
     for (int s = 0; s < d.spe; ++s) {
         const uint32_t sweep = sweep0 + (uint32_t)s;
         lds_sync<NW>();
@@ -810,6 +878,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
             a_reg = (b != a && j == inext) ? a : a_nx;
         }
     }
+    }  // intervals
 
     if (lane == 0) reinterpret_cast<uint64_t *>(red_s + 140)[wid] = cells;
     lds_sync<NW>();
@@ -884,19 +953,27 @@ __device__ __forceinline__ int64_t wave_min64(int64_t v) {
 // LDS of an exchange workgroup staging up to cap replicas: sized by the pool, so a launch
 // does not ask for room for kExMaxRT replicas beside the sweep workgroups of other pools
 // (with 50 KB a serving bank's exchange waited for CUs to free: 43 us per launch).
-__host__ __device__ inline int exchange_lds_bytes(int cap) { return 24 * cap + 1024; }
+__host__ __device__ inline int exchange_lds_bytes(int cap) { return 24 * cap; }
 
-__global__ __launch_bounds__(64) void exchange_kernel(PoolDev P, int cap) {
-    extern __shared__ __align__(16) unsigned char ex_smem[];
+// sum over the wave, every lane gets it
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) v += (uint32_t)__shfl_xor((int)v, off);
+    return v;
+}
+
+// One instance's exchange step (CPU_PIM.cpp:295-357), run by one wave (lane 0..63):
+// fold of the replica bests into the ladder bests, the bubble pass, the kick, the goal test
+// and the instance's interval bookkeeping.  With cap > 0 the energies, bests, ladder order
+// and positions of up to cap replicas are staged through LDS at ex_smem (24 bytes per
+// replica, coalesced loads issued in one round); with cap == 0 they are read and written in
+// global memory directly (the persistent sweep kernel, whose LDS holds its replica).
+__device__ void exchange_instance(const PoolDev &P, int inst, int cap, unsigned char *ex_smem,
+                                  int tid) {
     int64_t *E_s = reinterpret_cast<int64_t *>(ex_smem);
     int64_t *bE_s = E_s + cap;
-    int64_t *gbest_s = bE_s + cap;  // [64]
-    int32_t *ord_s = reinterpret_cast<int32_t *>(gbest_s + 64);
+    int32_t *ord_s = reinterpret_cast<int32_t *>(bE_s + cap);
     int32_t *pos_s = ord_s + cap;
-    uint32_t *nex_s = reinterpret_cast<uint32_t *>(pos_s + cap);  // [64]
-    uint32_t *nkick_s = nex_s + 64;                                // [64]
-    const int inst = blockIdx.x;
-    const int tid = threadIdx.x;
     // Every load of the step is issued first, in one round (all of it is valid memory
     // whether or not the instance still runs); the decision to run comes after.
     const InstDesc d = P.inst[inst];
@@ -921,7 +998,7 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P, int cap) {
         tss_pf = P.tss[d.ladder_base + tid];
         lbest_pf = P.ladder_best[d.ladder_base + tid];
     }
-    __syncthreads();
+    lds_sync<1>();
     if (st.halted || st.megas_done >= st.mega_limit) {
         if (tid == 0) P.state[inst].active = 0;
         return;
@@ -980,10 +1057,10 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P, int cap) {
             }
             gbest = lbest < gbest ? lbest : gbest;
         }
-        if (tid != 0) {  // the counts and best are wave-uniform: lane 0 reports them
-            gbest = INT64_MAX;
-            nex = nkick = 0;
-        }
+        // the counts and best are wave-uniform: lane 0 reports them
+        gbest = readlane64(gbest, 0);
+        nex = (uint32_t)__builtin_amdgcn_readlane((int)nex, 0);
+        nkick = (uint32_t)__builtin_amdgcn_readlane((int)nkick, 0);
     } else {
         for (int t = tid; t < T; t += 64) {
             const int L = d.ladder_base + t;
@@ -1020,11 +1097,11 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P, int cap) {
             P.ladder_best[L] = lbest;
             gbest = lbest < gbest ? lbest : gbest;
         }
+        gbest = wave_min64(gbest);
+        nex = wave_sum_u32(nex);
+        nkick = wave_sum_u32(nkick);
     }
-    gbest_s[tid] = gbest;
-    nex_s[tid] = nex;
-    nkick_s[tid] = nkick;
-    __syncthreads();
+    lds_sync<1>();
     if (staged) {
         for (int x = tid; x < RT; x += 64) {
             P.order[d.rep_base + x] = ord_s[x];
@@ -1032,17 +1109,11 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P, int cap) {
         }
     }
     if (tid == 0) {
-        int64_t gb = st.global_best;
-        uint64_t ne = 0, nk = 0;
-        for (int x = 0; x < 64; ++x) {
-            gb = gbest_s[x] < gb ? gbest_s[x] : gb;
-            ne += nex_s[x];
-            nk += nkick_s[x];
-        }
+        const int64_t gb = gbest < st.global_best ? gbest : st.global_best;
         InstState &so = P.state[inst];
         so.global_best = gb;
-        so.exchanges = st.exchanges + ne;
-        so.kicks = st.kicks + nk;
+        so.exchanges = st.exchanges + nex;
+        so.kicks = st.kicks + nkick;
         if (d.has_goal && gb <= d.goal) {
             so.halted = 1;
             so.active = 0;
@@ -1054,30 +1125,73 @@ __global__ __launch_bounds__(64) void exchange_kernel(PoolDev P, int cap) {
     }
 }
 
+__global__ __launch_bounds__(64) void exchange_kernel(PoolDev P, int cap) {
+    extern __shared__ __align__(16) unsigned char ex_smem[];
+    exchange_instance(P, blockIdx.x, cap, ex_smem, threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+// max_iv > 1: a persistent launch of max_iv exchange intervals, cooperative so that every
+// workgroup is resident (hipErrorCooperativeLaunchTooLarge when the device cannot hold them)
+// max_iv == 0: nothing is launched; the return says whether nblocks workgroups of the
+// kernel fit on the device at once (hipSuccess) or not (hipErrorCooperativeLaunchTooLarge).
 template <int NT, int CPL, int IM, bool SYM>
 static hipError_t launch_sweep_t(const PoolDev &P, bool sparse, const int32_t *list, int nblocks,
-                                 int lds_bytes, hipStream_t s) {
+                                 int lds_bytes, int max_iv, hipStream_t s) {
     auto k = sparse ? sweep_kernel<NT, CPL, IM, SYM, true> : sweep_kernel<NT, CPL, IM, SYM, false>;
     const hipError_t e = sparse ? allow_full_lds<sweep_kernel<NT, CPL, IM, SYM, true>>()
                                 : allow_full_lds<sweep_kernel<NT, CPL, IM, SYM, false>>();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds_bytes, s, P, list);
-    return hipGetLastError();
+    if (max_iv == 0) {
+        int per_cu = 0;
+        const hipError_t q = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(k), NT, (size_t)lds_bytes);
+        if (q != hipSuccess) return q;
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return hipErrorInvalidDevice;
+        return (int64_t)per_cu * cus >= nblocks ? hipSuccess : hipErrorCooperativeLaunchTooLarge;
+    }
+    if (max_iv <= 1) {
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds_bytes, s, P, list, 1);
+        return hipGetLastError();
+    }
+    PoolDev pd = P;
+    const int32_t *lp = list;
+    int iv = max_iv;
+    void *args[] = {&pd, &lp, &iv};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k), dim3(nblocks), dim3(NT),
+                                      args, (unsigned)lds_bytes, s);
 }
 
 template <int NT, int CPL>
 static hipError_t launch_sweep_c(const PoolDev &P, const SweepShape &sh, const int32_t *list,
-                                 int nblocks, int lds_bytes, hipStream_t s) {
+                                 int nblocks, int lds_bytes, int max_iv, hipStream_t s) {
     switch (sh.im * 2 + (sh.sym ? 1 : 0)) {
-    case IM_LDS32 * 2 + 1: return launch_sweep_t<NT, CPL, IM_LDS32, true>(P, sh.sparse, list, nblocks, lds_bytes, s);
-    case IM_LDS32 * 2 + 0: return launch_sweep_t<NT, CPL, IM_LDS32, false>(P, sh.sparse, list, nblocks, lds_bytes, s);
-    case IM_LDS16 * 2 + 1: return launch_sweep_t<NT, CPL, IM_LDS16, true>(P, sh.sparse, list, nblocks, lds_bytes, s);
-    case IM_LDS16 * 2 + 0: return launch_sweep_t<NT, CPL, IM_LDS16, false>(P, sh.sparse, list, nblocks, lds_bytes, s);
-    case IM_HBM * 2 + 1: return launch_sweep_t<NT, CPL, IM_HBM, true>(P, sh.sparse, list, nblocks, lds_bytes, s);
-    case IM_HBM * 2 + 0: return launch_sweep_t<NT, CPL, IM_HBM, false>(P, sh.sparse, list, nblocks, lds_bytes, s);
+    case IM_LDS32 * 2 + 1: return launch_sweep_t<NT, CPL, IM_LDS32, true>(P, sh.sparse, list, nblocks, lds_bytes, max_iv, s);
+    case IM_LDS32 * 2 + 0: return launch_sweep_t<NT, CPL, IM_LDS32, false>(P, sh.sparse, list, nblocks, lds_bytes, max_iv, s);
+    case IM_LDS16 * 2 + 1: return launch_sweep_t<NT, CPL, IM_LDS16, true>(P, sh.sparse, list, nblocks, lds_bytes, max_iv, s);
+    case IM_LDS16 * 2 + 0: return launch_sweep_t<NT, CPL, IM_LDS16, false>(P, sh.sparse, list, nblocks, lds_bytes, max_iv, s);
+    case IM_HBM * 2 + 1: return launch_sweep_t<NT, CPL, IM_HBM, true>(P, sh.sparse, list, nblocks, lds_bytes, 1, s);
+    case IM_HBM * 2 + 0: return launch_sweep_t<NT, CPL, IM_HBM, false>(P, sh.sparse, list, nblocks, lds_bytes, 1, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+static hipError_t launch_sweep_iv(const PoolDev &P, const SweepShape &sh, const int32_t *list,
+                                  int nblocks, int lds_bytes, int max_iv, hipStream_t s) {
+    switch (sh.nt * 8 + sh.cpl) {
+    case 64 * 8 + 1: return launch_sweep_c<64, 1>(P, sh, list, nblocks, lds_bytes, max_iv, s);
+    case 64 * 8 + 2: return launch_sweep_c<64, 2>(P, sh, list, nblocks, lds_bytes, max_iv, s);
+    case 64 * 8 + 4: return launch_sweep_c<64, 4>(P, sh, list, nblocks, lds_bytes, max_iv, s);
+    case 128 * 8 + 1: return launch_sweep_c<128, 1>(P, sh, list, nblocks, lds_bytes, max_iv, s);
+    case 128 * 8 + 2: return launch_sweep_c<128, 2>(P, sh, list, nblocks, lds_bytes, max_iv, s);
+    case 256 * 8 + 1: return launch_sweep_c<256, 1>(P, sh, list, nblocks, lds_bytes, max_iv, s);
+    case 256 * 8 + 2: return launch_sweep_c<256, 2>(P, sh, list, nblocks, lds_bytes, max_iv, s);
+    case 256 * 8 + 4: return launch_sweep_c<256, 4>(P, sh, list, nblocks, lds_bytes, max_iv, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1087,17 +1201,19 @@ hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *l
     if (nblocks <= 0) return hipSuccess;
     if (sh.im == IM_LAZY) return launch_sweep_lazy(P, sh, list, nblocks, lds_bytes, s);
     if (sh.im == IM_ONESHOT) return launch_oneshot(P, sh, list, nblocks, lds_bytes, s);
-    switch (sh.nt * 8 + sh.cpl) {
-    case 64 * 8 + 1: return launch_sweep_c<64, 1>(P, sh, list, nblocks, lds_bytes, s);
-    case 64 * 8 + 2: return launch_sweep_c<64, 2>(P, sh, list, nblocks, lds_bytes, s);
-    case 64 * 8 + 4: return launch_sweep_c<64, 4>(P, sh, list, nblocks, lds_bytes, s);
-    case 128 * 8 + 1: return launch_sweep_c<128, 1>(P, sh, list, nblocks, lds_bytes, s);
-    case 128 * 8 + 2: return launch_sweep_c<128, 2>(P, sh, list, nblocks, lds_bytes, s);
-    case 256 * 8 + 1: return launch_sweep_c<256, 1>(P, sh, list, nblocks, lds_bytes, s);
-    case 256 * 8 + 2: return launch_sweep_c<256, 2>(P, sh, list, nblocks, lds_bytes, s);
-    case 256 * 8 + 4: return launch_sweep_c<256, 4>(P, sh, list, nblocks, lds_bytes, s);
-    default: return hipErrorInvalidValue;
-    }
+    return launch_sweep_iv(P, sh, list, nblocks, lds_bytes, 1, s);
+}
+
+hipError_t launch_sweep_persistent(const PoolDev &P, const SweepShape &sh, const int32_t *list,
+                                   int nblocks, int lds_bytes, int intervals, hipStream_t s) {
+    if (nblocks <= 0) return hipSuccess;
+    if (sh.im != IM_LDS32 && sh.im != IM_LDS16) return hipErrorInvalidValue;
+    return launch_sweep_iv(P, sh, list, nblocks, lds_bytes, intervals < 1 ? 1 : intervals, s);
+}
+
+bool sweep_persistent_fits(const SweepShape &sh, int nblocks, int lds_bytes) {
+    if (sh.im != IM_LDS32 && sh.im != IM_LDS16) return false;
+    return launch_sweep_iv(PoolDev{}, sh, nullptr, nblocks, lds_bytes, 0, nullptr) == hipSuccess;
 }
 
 bool sweep_shape_supported(int nt, int cpl, int im) {

@@ -341,15 +341,64 @@ def test_batch_mixes_one_shot_and_sweep_instances(qpt, oracle, gpu):
 
 
 def test_timed_mode_same_trajectory(qpt, oracle, gpu):
+    """Timed runs (events around every sweep launch) follow the same trajectory: one
+    persistent launch for the 8 intervals of this latency-bound LDS pool, or 8 launches
+    with persistent launches switched off."""
     F, D = qpt.instances.flat_c1()
     betas = qpt.instances.geometric_ladder(16, 1.0, 100.0)
-    inst = qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=8, seed=4)
-    with qpt.Pool([inst]) as pool:
-        ms, launches = pool.run(-1, timed=True)
-        st = pool.state(0)
-    assert launches == 8 and ms > 0
     ref = oracle.solve(F, D, betas, 8, seed=4)
-    np.testing.assert_array_equal(st["perm"], ref["final_perm"])
+    for persist, want in (("1", 1), ("0", 8)):
+        os.environ["QAP_PT_PERSIST"] = persist
+        try:
+            inst = qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=8, seed=4)
+            with qpt.Pool([inst]) as pool:
+                ms, launches = pool.run(-1, timed=True)
+                st = pool.state(0)
+        finally:
+            os.environ.pop("QAP_PT_PERSIST", None)
+        assert launches == want and ms > 0, (persist, launches)
+        np.testing.assert_array_equal(st["perm"], ref["final_perm"])
+        np.testing.assert_array_equal(st["E"], ref["final_E"])
+
+
+PERSIST_CASES = {
+    # latency-bound LDS pools: intervals inside one launch vs one launch per interval
+    "flat_c1_R16": (lambda q: q.instances.flat_c1(), 16, 60, {}),
+    "grey_R8_T2_spe2_lds16": (lambda q: q.instances.grey_c2(), 8, 6, {"T": 2, "spe": 2}),
+    "asym_n45_lds16_kick": (lambda q: q.instances.random_instance(23, 45, symmetric=False,
+                                                                  zero_diag=False), 6, 40,
+                            {"kick_after": 2, "env": {"QAP_PT_IMODE": "1"}}),
+    "long_ladder_R70_T3": (lambda q: q.instances.random_instance(33, 16), 70, 12,
+                           {"T": 3, "kick_after": 2}),
+    "n100_256x1_T12": (lambda q: q.instances.mixed_instance(5, 100, True), 4, 10,
+                       {"T": 12, "env": {"QAP_PT_SHAPE": "256,1"}}),
+}
+
+
+@pytest.mark.parametrize("name", list(PERSIST_CASES))
+def test_persistent_launch_equals_interval_launches(qpt, oracle, gpu, name):
+    """The exchange step inside a persistent launch (last replica of the instance to arrive)
+    gives the same trajectory as exchange launches between sweep launches, and both equal
+    the oracle's; a goal stops both at the same interval."""
+    build, R, sweeps, kw = PERSIST_CASES[name]
+    F, D = build(qpt)
+    n = F.shape[0]
+    betas = qpt.instances.geometric_ladder(R, 1.0, 100.0) / max(1.0, n / 16.0)
+    print(f"persist case {name}: n={n} R={R} sweeps={sweeps} kw={kw}", flush=True)
+    env = dict(kw.pop("env", {}))
+    out = {}
+    for persist in ("1", "0"):
+        res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, sweeps,
+                                      env=dict(env, QAP_PT_PERSIST=persist), **kw)
+        _assert_same(res, st, ver, ref)
+        out[persist] = (res, st)
+    free = oracle.solve(F, D, betas, sweeps, seed=7, **kw)
+    goal = int(free["best_cost"] * 1.01)
+    for persist in ("1", "0"):
+        res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, sweeps, goal=goal,
+                                      env=dict(env, QAP_PT_PERSIST=persist), **kw)
+        assert res["sweeps_done"] == ref["megas_done"] * kw.get("spe", 1)
+        _assert_same(res, st, ver, ref)
 
 
 def test_frozen_ladder_all_weights_zero(qpt, oracle, gpu):
