@@ -36,10 +36,13 @@ struct TunedShape {
     int n_max, sparse, im, nt, cpl;
     double rate;
 };
-// The tuned row for an instance of size n in placement im, or nullptr.  The table is read once per
-// process from $QAP_PT_TUNE_TABLE (empty: no table) or else from
-// <directory of libqap_pt.so>/../tune/shapes_gfx950.txt; a missing file means no table.
-const TunedShape *tuned_shape(int n, bool sparse, int im);
+// The tuned row for an instance of size n in placement im, or nullptr: from the full-chip
+// table, or with `latency` from the table of latency-bound pools (at most one replica per
+// CU).  The tables are read once per process from $QAP_PT_TUNE_TABLE and
+// $QAP_PT_TUNE_LATENCY_TABLE (empty: no table) or else from <directory of
+// libqap_pt.so>/../tune/shapes_gfx950.txt and shapes_latency_gfx950.txt; a missing file
+// means no table.
+const TunedShape *tuned_shape(int n, bool sparse, int im, bool latency = false);
 
 // Slot mode (qap_pt_slots.cpp): a pool of `slots` fixed-capacity instance slots that the
 // serving loop loads and releases between launches.  Every call except slots_launch

@@ -126,7 +126,7 @@ bool throughput_regime(int64_t replicas) { return replicas > device_cus(); }
 
 // Threads x candidates per thread for placement s.im at size n: the tuned table's row for
 // (n, flow class, placement), else a size rule; QAP_PT_SHAPE="nt,cpl" overrides both.
-void shape_for_size(int n, bool sparse, SweepShape &s) {
+void shape_for_size(int n, bool sparse, SweepShape &s, bool latency) {
     if (n <= 64) {
         s.nt = 64;
         s.cpl = 1;
@@ -153,7 +153,10 @@ void shape_for_size(int n, bool sparse, SweepShape &s) {
             s.cpl = cpl;
         }
     };
-    if (const TunedShape *t = tuned_shape(n, sparse, s.im)) take(t->nt, t->cpl);
+    // a latency-bound pool's own row first, else the full-chip row
+    const TunedShape *t = latency ? tuned_shape(n, sparse, s.im, true) : nullptr;
+    if (!t) t = tuned_shape(n, sparse, s.im, false);
+    if (t) take(t->nt, t->cpl);
     if (const char *env = std::getenv("QAP_PT_SHAPE")) {
         int nt = 0, cpl = 0;
         if (std::sscanf(env, "%d,%d", &nt, &cpl) == 2) take(nt, cpl);
@@ -176,10 +179,10 @@ namespace {
 // Threads x candidates per thread and force-matrix placement for size n.
 // Placement: QAP_PT_IMODE=0|1|2|3 when that one fits (LDS budget, uint16 range, lazy
 // conditions); else, for a throughput-bound pool, the fitting placement with the highest
-// tuned rate among those the table rates; else the first of lds32 -> lds16 -> lazy -> hbm
-// that fits.  Shape: QAP_PT_SHAPE="nt,cpl" (tests and
-// the autotuner), else the tuned table's row for (n, flow class, placement), else a size
-// rule.
+// rate in the full-chip table among those it rates; else the first of lds32 -> lds16 ->
+// lazy -> hbm that fits.  Shape: QAP_PT_SHAPE="nt,cpl" (tests and the autotuner), else
+// the row for (n, flow class, placement) of the pool's regime table (latency-bound pools:
+// the latency table, then the full-chip one), else a size rule.
 SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool sparse,
                         bool small_uv, bool throughput, int *ld_s) {
     SweepShape s{};
@@ -204,8 +207,12 @@ SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool 
         // size bucket is rated at its largest n, so an LDS placement that fits only below
         // that size has no rate there; it drops out (it needs the most LDS, the slowest
         // at full load: n = 161..186 in an n <= 256 bank ran the int32 LDS kernel at a
-        // third of the lazy kernel's rate).  A smaller pool is latency-bound and keeps
-        // the order above, which puts the force matrix in LDS whenever it fits.
+        // third of the lazy kernel's rate).  A latency-bound pool keeps the order above
+        // (force matrix in LDS whenever it fits): with a CU per replica the placements'
+        // rates follow the acceptance rate and the flow structure more than n (the
+        // latency table rates lazy above the LDS placements from n = 96 on its random
+        // binary-flow instances, while the grey-pattern n = 256 instance runs 57 M
+        // swap-attempts/s in uint16 LDS against 23 M lazy), so that table sets shapes only.
         const int cand[4] = {fit32 ? IM_LDS32 : -1, fit16 ? IM_LDS16 : -1, lazy ? IM_LAZY : -1,
                              IM_HBM};
         double best_rate = 0.0;
@@ -221,7 +228,7 @@ SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool 
         if (best_im >= 0) s.im = best_im;
     }
     *ld_s = lds_stride(n, s.im);
-    shape_for_size(n, sparse, s);
+    shape_for_size(n, sparse, s, !throughput);
     return s;
 }
 

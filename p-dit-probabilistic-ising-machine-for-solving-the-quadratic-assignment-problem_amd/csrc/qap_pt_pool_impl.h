@@ -145,7 +145,7 @@ int device_cus();
 bool throughput_regime(int64_t replicas);
 // Threads x candidates per thread for placement s.im at size n (tuned table, size rule,
 // QAP_PT_SHAPE override); sets s.nt and s.cpl.
-void shape_for_size(int n, bool sparse, SweepShape &s);
+void shape_for_size(int n, bool sparse, SweepShape &s, bool latency = false);
 // Orders a bucket's workgroups: larger n first, groups of 8 instances interleaved
 // replica by replica (consecutive workgroup ids go round-robin over the 8 XCDs, so each
 // instance stays on one XCD and its F and D stay in that XCD's L2).

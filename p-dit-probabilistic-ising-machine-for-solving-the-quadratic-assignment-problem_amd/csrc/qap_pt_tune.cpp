@@ -1,5 +1,8 @@
-// qap_pt_tune.cpp -- the tuned sweep-shape table (written by the package's autotune.py
-// on the device it describes; read here by the shape choice at pool creation).
+// qap_pt_tune.cpp -- the tuned sweep-shape tables (written by the package's autotune.py
+// on the device it describes; read here by the shape choice at pool creation): one for
+// full-chip loads (shapes_gfx950.txt, rates of kernel time with many replicas per CU) and
+// one for latency-bound pools of at most one replica per CU (shapes_latency_gfx950.txt,
+// rates of wall time through the launches such a pool runs).
 //
 // Format, one row per line, '#' starts a comment:
 //   n_max  dense|sparse  lds32|lds16|lazy|hbm  nt  cpl  [swap-attempts/s]
@@ -22,21 +25,21 @@
 namespace qap {
 namespace {
 
-std::vector<TunedShape> g_table;
+std::vector<TunedShape> g_table[2];  // [0] full-chip, [1] latency-bound
 std::once_flag g_table_once;
 
-std::string default_table_path() {
+std::string default_table_path(const char *file) {
     Dl_info info{};
     if (!dladdr(reinterpret_cast<void *>(&tuned_shape), &info) || !info.dli_fname) return {};
     std::string lib = info.dli_fname;
     const size_t slash = lib.rfind('/');
     const std::string dir = slash == std::string::npos ? std::string(".") : lib.substr(0, slash);
-    return dir + "/../tune/shapes_gfx950.txt";
+    return dir + "/../tune/" + file;
 }
 
-void load_table() {
-    const char *env = std::getenv("QAP_PT_TUNE_TABLE");
-    const std::string path = env ? std::string(env) : default_table_path();
+void load_table(std::vector<TunedShape> &table, const char *env_name, const char *file) {
+    const char *env = std::getenv(env_name);
+    const std::string path = env ? std::string(env) : default_table_path(file);
     if (path.empty()) return;
     std::ifstream in(path);
     if (!in.is_open()) return;
@@ -57,17 +60,22 @@ void load_table() {
         else if (place == "lazy") t.im = IM_LAZY;
         else continue;
         if (t.n_max < 1 || !sweep_shape_supported(t.nt, t.cpl, t.im)) continue;
-        g_table.push_back(t);
+        table.push_back(t);
     }
-    std::stable_sort(g_table.begin(), g_table.end(),
+    std::stable_sort(table.begin(), table.end(),
                      [](const TunedShape &a, const TunedShape &b) { return a.n_max < b.n_max; });
+}
+
+void load_tables() {
+    load_table(g_table[0], "QAP_PT_TUNE_TABLE", "shapes_gfx950.txt");
+    load_table(g_table[1], "QAP_PT_TUNE_LATENCY_TABLE", "shapes_latency_gfx950.txt");
 }
 
 }  // namespace
 
-const TunedShape *tuned_shape(int n, bool sparse, int im) {
-    std::call_once(g_table_once, load_table);
-    for (const TunedShape &t : g_table)
+const TunedShape *tuned_shape(int n, bool sparse, int im, bool latency) {
+    std::call_once(g_table_once, load_tables);
+    for (const TunedShape &t : g_table[latency ? 1 : 0])
         if (t.sparse == (sparse ? 1 : 0) && t.im == im && n <= t.n_max) return &t;
     return nullptr;
 }

@@ -1,7 +1,8 @@
-"""The tuned shape table: parsing and precedence (host only, no device).
+"""The tuned shape tables: parsing and precedence (host only, no device).
 
-The library reads the table once per process, so each case runs in a fresh Python
-process with QAP_PT_TUNE_TABLE pointing at a file written here.
+The library reads the tables once per process, so each case runs in a fresh Python
+process with QAP_PT_TUNE_TABLE (full-chip) and QAP_PT_TUNE_LATENCY_TABLE (latency-bound
+pools) pointing at files written here, or empty.
 """
 import os
 import subprocess
@@ -21,16 +22,18 @@ PROBE = textwrap.dedent("""
 """)
 
 
-def _shapes(tmp_path, table_text, extra_env=None):
+def _shapes(tmp_path, table_text, extra_env=None, latency_text=None):
     env = dict(os.environ)
     env.pop("QAP_PT_SHAPE", None)
     env.pop("QAP_PT_IMODE", None)
-    if table_text is None:
-        env["QAP_PT_TUNE_TABLE"] = ""
-    else:
-        path = tmp_path / "shapes.txt"
-        path.write_text(table_text)
-        env["QAP_PT_TUNE_TABLE"] = str(path)
+    for var, text, name in (("QAP_PT_TUNE_TABLE", table_text, "shapes.txt"),
+                            ("QAP_PT_TUNE_LATENCY_TABLE", latency_text, "latency.txt")):
+        if text is None:
+            env[var] = ""
+        else:
+            path = tmp_path / name
+            path.write_text(text)
+            env[var] = str(path)
     env.update(extra_env or {})
     out = subprocess.run([sys.executable, "-c", PROBE.format(tests=os.path.join(ROOT, "tests"))],
                          capture_output=True, text=True, env=env, timeout=120, check=True).stdout
@@ -105,3 +108,20 @@ def test_rates_choose_among_placements_that_fit(tmp_path):
     # an explicit placement request still wins
     s = _shapes(tmp_path, table, {"QAP_PT_IMODE": "0"})
     assert s["40@full"].startswith("64x1,lds32")
+
+
+def test_latency_table_sets_shapes_of_small_pools_only(tmp_path):
+    # a latency-bound pool (at most one replica per CU) takes its shape from the latency
+    # rows, falling back to the full-chip rows; its placement keeps the fixed order (the
+    # latency rates do not move it); a full-chip pool ignores the latency rows
+    latency = textwrap.dedent("""\
+        64 dense lds32 64 1 2.0e7
+        64 dense lazy 64 1 3.0e7
+        128 dense lds32 64 2 1.5e7
+        """)
+    full = "256 dense lds32 128 2\n"
+    s = _shapes(tmp_path, full, latency_text=latency)
+    assert s["40"].startswith("64x1,lds32")       # placement order, not the lazy rate
+    assert s["100"].startswith("64x2,lds32")      # the latency row's shape
+    assert s["129"].startswith("128x2,lds32")     # no latency row: the full-chip row
+    assert s["100@full"].startswith("128x2,lds32")  # full-chip pools: full-chip rows only
