@@ -163,13 +163,17 @@ def roofline_pass(pool, insts, sweeps, traffic_name):
     placement = key.split(",")[1]
     lds = placement in ("lds32", "lds16")
     peak = LDS_PEAK_GBS if lds else HBM_PEAK_GBS
+    # HBM bytes per launch from the PMC record of this workload and kernel, kept per
+    # exchange interval (a persistent launch runs many intervals)
+    spe = max(1, insts[0].sweeps_per_exchange)
+    intervals_per_launch = sweeps / spe / launches
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{traffic_name}.txt")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             rec = json.load(fh)
-        if rec.get("shape", best["shape"]) == best["shape"]:
-            traffic = rec.get("hbm_bytes_per_launch")
+        if rec.get("shape") == best["shape"] and "hbm_bytes_per_interval" in rec:
+            traffic = rec["hbm_bytes_per_interval"] * intervals_per_launch
     out = {
         "bound": "lds" if lds else "hbm",
         "achieved": achieved,
@@ -180,6 +184,7 @@ def roofline_pass(pool, insts, sweeps, traffic_name):
         "kernel": best["shape"],
         "kernel_avg_us": avg_s * 1e6,
         "kernel_launches": best["launches"],
+        "intervals_per_launch": intervals_per_launch,
         "instances_in_kernel": members,
         "algorithmic_bytes_per_launch": alg / launches,
         "peak_basis": ("LDS: 256 CUs x 128 B/clk x 2.4 GHz" if lds else "HBM3E 8 TB/s"),
