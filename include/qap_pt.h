@@ -20,9 +20,11 @@
  * message for qap_pt_last_error() (thread-local).  The caller owns every buffer;
  * inputs are only read during the call.  Solve calls are reentrant; one pool or
  * server handle must not be used from two threads at once.  No C++ exception
- * crosses this boundary.  All costs are exact integers (int64); force matrices are
- * int32 and a permutation-independent bound is checked before any launch
- * (QAP_PT_E_OVERFLOW when it cannot be proven).
+ * crosses this boundary.  All costs are exact integers (int64).  Force matrices are
+ * int32 when a permutation-independent bound proves the deltas fit (4 B + 16 maxD maxF <
+ * 2^30, B the force bound), else int64 in HBM (GPU-CUPIM's ReplicaPool<F, int64_t>,
+ * qap.cu:369-411, --large-energies at qap_gpu.cu:789-793) when 4 B + 16 maxD maxF < 2^62
+ * and n^2 maxD maxF < 2^62; beyond that, QAP_PT_E_OVERFLOW before any launch.
  */
 #ifndef QAP_PT_H
 #define QAP_PT_H

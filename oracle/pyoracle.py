@@ -63,6 +63,8 @@ def lib() -> C.CDLL:
         L.oracle_philox4x32_10.argtypes = [_U32P, _U32P, _U32P]
         L.oracle_weight.argtypes = [C.c_float, C.c_int32]
         L.oracle_weight.restype = C.c_uint32
+        L.oracle_weight64.argtypes = [C.c_float, C.c_int64]
+        L.oracle_weight64.restype = C.c_uint32
         L.oracle_force_matrix.argtypes = [C.c_int32, _I32P, _I32P, _I32P, _I32P]
         L.oracle_energy.argtypes = [C.c_int32, _I32P, _I32P, _I32P]
         L.oracle_energy.restype = C.c_int64
@@ -107,6 +109,10 @@ def philox(ctr, key):
 
 def weight(beta2f: float, d: int) -> int:
     return int(lib().oracle_weight(C.c_float(beta2f), int(d)))
+
+
+def weight64(beta2f: float, d: int) -> int:
+    return int(lib().oracle_weight64(C.c_float(beta2f), int(d)))
 
 
 def beta2(beta_ln: float) -> float:

@@ -73,8 +73,8 @@ static const float EXP2_C[8] = {
     0x1.3b2ab6p-7f,  0x1.5d87fep-10f, 0x1.430912p-13f, 0x1.ffcbfcp-17f,
 };
 
-uint32_t oracle_weight(float beta2f, int32_t d) {
-    float y = beta2f * (float)d;
+/* floor(2^20 * 2^y) for y = beta2f * d: the float steps after the product */
+static uint32_t oracle_weight_y(float y) {
     if (!(y > -32.0f)) return 0u;
     float kf = rintf(y);
     float f = y - kf;
@@ -85,6 +85,8 @@ uint32_t oracle_weight(float beta2f, int32_t d) {
     float w = p * (float)(1u << e);
     return (uint32_t)w;
 }
+
+uint32_t oracle_weight(float beta2f, int32_t d) { return oracle_weight_y(beta2f * (float)d); }
 
 double oracle_beta2(double beta_ln) { return beta_ln * 1.44269504089; }
 
@@ -166,10 +168,29 @@ void oracle_bounds(int32_t n, const int32_t *F, const int32_t *D, int64_t *force
         if (rd > sD) sD = rd;
         if (rf > sF) sF = rf;
     }
-    int64_t b1 = maxF * sD, b2 = maxD * sF;
-    int64_t B = b1 < b2 ? b1 : b2;
-    *force_bound = B;
-    *delta_bound = 4 * B + 16 * maxD * maxF;
+    /* products in 128 bits: with entries near 2^31 the bounds pass 2^63; they saturate
+     * at INT64_MAX (such an instance is refused) */
+    __int128 b1 = (__int128)maxF * sD, b2 = (__int128)maxD * sF;
+    __int128 B = b1 < b2 ? b1 : b2;
+    __int128 db = 4 * B + 16 * (__int128)maxD * maxF;
+    *force_bound = B > INT64_MAX ? INT64_MAX : (int64_t)B;
+    *delta_bound = db > INT64_MAX ? INT64_MAX : (int64_t)db;
+}
+
+/* 0: the force matrix, deltas and update terms provably fit int32 (the int32 path);
+ * 1: they fit int64 only (the wide path: 4 B + 16 maxD maxF < 2^62 and the cost
+ *    n^2 maxD maxF < 2^62); -1: not even that. */
+static int32_t width_class(int32_t n, const int32_t *F, const int32_t *D) {
+    int64_t fb, db, maxD = 0, maxF = 0;
+    oracle_bounds(n, F, D, &fb, &db);
+    for (int64_t x = 0; x < (int64_t)n * n; x++) {
+        int64_t d = llabs((long long)D[x]), f = llabs((long long)F[x]);
+        if (d > maxD) maxD = d;
+        if (f > maxF) maxF = f;
+    }
+    if (db < ((int64_t)1 << 30) && fb + 8 * maxD * maxF < ((int64_t)1 << 31)) return 0;
+    if (db < ((int64_t)1 << 62) && (__int128)n * n * maxD * maxF < ((__int128)1 << 62)) return 1;
+    return -1;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -332,6 +353,131 @@ static int32_t attempt_reffp(int32_t n, const int32_t *F, const int32_t *D, int3
 }
 
 /* ------------------------------------------------------------------------- */
+/* The wide path: the same update with an int64 force matrix, for instances whose   */
+/* bounds exceed int32 (CUPIM's ReplicaPool<F, int64_t>, qap.cu:369-411 with        */
+/* Z = int64_t).  Deltas, cutoff and the weight's input are int64; the weight      */
+/* itself is the same float sequence from y = beta2f * (float)d on.                */
+/* ------------------------------------------------------------------------- */
+static void force_matrix64(int32_t n, const int32_t *F, const int32_t *D, const int32_t *perm,
+                           int64_t *I) {
+    int32_t *Ft = malloc(sizeof(int32_t) * (size_t)n * n);
+    for (int x = 0; x < n; x++)
+        for (int y = 0; y < n; y++) Ft[(size_t)x * n + y] = F[(size_t)y * n + x];
+    for (int i = 0; i < n; i++) {
+        int64_t *row = I + (size_t)i * n;
+        for (int a = 0; a < n; a++) row[a] = 0;
+        for (int j = 0; j < n; j++) {
+            const int64_t dij = D[(size_t)i * n + j], dji = D[(size_t)j * n + i];
+            const int32_t *fcol = Ft + (size_t)perm[j] * n, *frow = F + (size_t)perm[j] * n;
+            for (int a = 0; a < n; a++) row[a] -= dij * fcol[a] + dji * frow[a];
+        }
+    }
+    free(Ft);
+}
+
+static int64_t delta64(int32_t n, const int32_t *F, const int32_t *D, const int32_t *perm,
+                       const int32_t *pinv, const int64_t *I, int32_t i, int32_t b) {
+    const int32_t a = perm[i], j = pinv[b];
+    const int64_t force = I[(size_t)i * n + b] + I[(size_t)j * n + a] - I[(size_t)i * n + a] -
+                          I[(size_t)j * n + b];
+    const int64_t dist = (int64_t)D[i * n + i] + D[j * n + j] - D[i * n + j] - D[j * n + i];
+    const int64_t flow = (int64_t)F[a * n + a] + F[b * n + b] - F[a * n + b] - F[b * n + a];
+    return force - dist * flow;
+}
+
+static void commit64(int32_t n, const int32_t *F, const int32_t *D, int32_t *perm, int32_t *pinv,
+                     int64_t *I, int64_t *E, int32_t i, int32_t b, int64_t dEb) {
+    const int32_t a = perm[i], j = pinv[b];
+    *E -= dEb;
+    perm[i] = b;
+    pinv[b] = i;
+    perm[j] = a;
+    pinv[a] = j;
+    int64_t *v = malloc(sizeof(int64_t) * 2 * (size_t)n);
+    for (int c = 0; c < n; c++) {
+        v[c] = (int64_t)F[(size_t)c * n + b] - F[(size_t)c * n + a];
+        v[n + c] = (int64_t)F[(size_t)b * n + c] - F[(size_t)a * n + c];
+    }
+    for (int k = 0; k < n; k++) {
+        const int64_t u1 = (int64_t)D[(size_t)k * n + j] - D[(size_t)k * n + i];
+        const int64_t u2 = (int64_t)D[(size_t)j * n + k] - D[(size_t)i * n + k];
+        int64_t *row = I + (size_t)k * n;
+        for (int c = 0; c < n; c++) row[c] += u1 * v[c] + u2 * v[n + c];
+    }
+    free(v);
+}
+
+/* cutoff = (int64)(maxd - 20/beta2), saturated below (CPU_PIM.cpp:234-235) */
+static int64_t cutoff64(int64_t maxd, double c20) {
+    const double cd = (double)maxd - c20;
+    if (cd < -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)cd;
+}
+
+uint32_t oracle_weight64(float beta2f, int64_t d) {
+    return oracle_weight_y(beta2f * (float)d);
+}
+
+static int32_t attempt_exact64(int32_t n, const int32_t *F, const int32_t *D, int32_t *perm,
+                               int32_t *pinv, int64_t *I, int64_t *E, int32_t i, float beta2f,
+                               double c20, uint32_t u, int64_t *dE, uint32_t *w) {
+    const int32_t a = perm[i];
+    int64_t maxd = 0;
+    for (int b = 0; b < n; b++) {
+        dE[b] = delta64(n, F, D, perm, pinv, I, i, b);
+        if (dE[b] > maxd) maxd = dE[b];
+    }
+    const int64_t cut = cutoff64(maxd, c20);
+    uint32_t S = 0;
+    for (int b = 0; b < n; b++) {
+        w[b] = dE[b] > cut ? oracle_weight64(beta2f, dE[b] - maxd) : 0u;
+        S += w[b];
+    }
+    const uint32_t T = (uint32_t)(((uint64_t)u * S) >> 32);
+    uint32_t P = 0;
+    int32_t sel = a;
+    for (int b = 0; b < n; b++) {
+        P += w[b];
+        if (P > T) {
+            sel = b;
+            break;
+        }
+    }
+    if (sel != a) commit64(n, F, D, perm, pinv, I, E, i, sel, dE[sel]);
+    return sel;
+}
+
+static int32_t attempt_reffp64(int32_t n, const int32_t *F, const int32_t *D, int32_t *perm,
+                               int32_t *pinv, int64_t *I, int64_t *E, int32_t i, double beta2,
+                               double r, int64_t *dE, long double *probs) {
+    const int32_t a = perm[i];
+    int64_t maxd = 0;
+    for (int b = 0; b < n; b++) {
+        dE[b] = delta64(n, F, D, perm, pinv, I, i, b);
+        if (dE[b] > maxd) maxd = dE[b];
+    }
+    const int64_t cut = cutoff64(maxd, 20.0 / beta2);
+    double probSum = 0.0;
+    for (int b = 0; b < n; b++) {
+        probs[b] = 0.0L;
+        if (dE[b] > cut) {
+            probs[b] = exp2l((long double)beta2 * (long double)dE[b]);
+            probSum += probs[b];
+        }
+    }
+    const double scaledR = r * probSum;
+    double cum = 0.0;
+    for (int b = 0; b < n; b++) {
+        cum += probs[b];
+        if (cum >= scaledR) {
+            if (b != a) commit64(n, F, D, perm, pinv, I, E, i, b, dE[b]);
+            return b;
+        }
+    }
+    return a;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Initial permutations (CPU_PIM.cpp:134-161).                                */
 /* ------------------------------------------------------------------------- */
 static void init_perm_tag(int32_t n, uint64_t seed, uint32_t stream, uint32_t tag,
@@ -358,6 +504,7 @@ void oracle_init_perm(int32_t n, uint64_t seed, uint32_t stream, int32_t *perm) 
 /* ------------------------------------------------------------------------- */
 typedef struct {
     int32_t *perm, *pinv, *I, *best_perm;
+    int64_t *I64;           /* the wide path's force matrix (I is then unused) */
     int64_t E, bestE;
     uint64_t accepted;
     int32_t pos;
@@ -377,6 +524,8 @@ typedef struct {
     const double *beta2;    /* per position */
     const double *c20;      /* per position */
     int32_t *dE;
+    int64_t *dE64;          /* wide path */
+    int32_t wide;
     uint32_t *w;
     int32_t *os_work;       /* 4n one-shot scratch */
     uint64_t *os_owner;     /* n */
@@ -464,7 +613,16 @@ static void ladder_sweeps(ladder_t *L, int64_t m) {
                 uint32_t x[4];
                 ph_draw(par->seed, (uint32_t)i, sweep, stream, 0u, x);
                 int32_t a = rp->perm[i], sel;
-                if (L->mode == ORACLE_MODE_EXACT) {
+                if (L->wide && L->mode == ORACLE_MODE_EXACT) {
+                    sel = attempt_exact64(n, P->F, P->D, rp->perm, rp->pinv, rp->I64, &rp->E, i,
+                                          L->beta2f[rp->pos], L->c20[rp->pos], x[0], L->dE64,
+                                          L->w);
+                } else if (L->wide) {
+                    double r = (double)((((uint64_t)(x[0] >> 5)) << 26) | (uint64_t)(x[1] >> 6)) *
+                               0x1p-53;
+                    sel = attempt_reffp64(n, P->F, P->D, rp->perm, rp->pinv, rp->I64, &rp->E, i,
+                                          L->beta2[rp->pos], r, L->dE64, L->probs);
+                } else if (L->mode == ORACLE_MODE_EXACT) {
                     sel = oracle_attempt_exact(n, P->F, P->D, rp->perm, rp->pinv, rp->I, &rp->E, i,
                                                L->beta2f[rp->pos], L->c20[rp->pos], x[0], L->dE,
                                                L->w);
@@ -514,9 +672,10 @@ int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_re
     if (par->sweeps > (int64_t)0xffffffff) return -1;
     for (int k = 0; k < R; k++)
         if (!(par->beta_ln[k] > 0.0)) return -1;
-    int64_t fb, db;
-    oracle_bounds(n, prob->F, prob->D, &fb, &db);
-    if (db >= ((int64_t)1 << 30)) return -4;
+    /* int32 force matrices when the bounds allow, else the wide (int64) path; one-shot
+     * steps need small entries anyway */
+    const int32_t wide = width_class(n, prob->F, prob->D);
+    if (wide < 0 || (wide && (par->flags & ORACLE_FLAG_ONE_SHOT))) return -4;
     const int32_t RT = R * T;
     const int64_t megas = par->sweeps / par->spe;
 
@@ -541,7 +700,8 @@ int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_re
             rp->perm = malloc(sizeof(int32_t) * n);
             rp->pinv = malloc(sizeof(int32_t) * n);
             rp->best_perm = malloc(sizeof(int32_t) * n);
-            rp->I = malloc(sizeof(int32_t) * (size_t)n * n);
+            if (wide) rp->I64 = malloc(sizeof(int64_t) * (size_t)n * n);
+            else rp->I = malloc(sizeof(int32_t) * (size_t)n * n);
             if (par->flags & ORACLE_FLAG_INIT_PER_REPLICA) {
                 if (par->init_perm) memcpy(rp->perm, par->init_perm, sizeof(int32_t) * n);
                 else init_perm_tag(n, par->seed, (uint32_t)(t * R + l), 2u, rp->perm);
@@ -550,7 +710,8 @@ int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_re
             }
             for (int i = 0; i < n; i++) rp->pinv[rp->perm[i]] = i;
             memcpy(rp->best_perm, rp->perm, sizeof(int32_t) * n);
-            oracle_force_matrix(n, prob->F, prob->D, rp->perm, rp->I);
+            if (wide) force_matrix64(n, prob->F, prob->D, rp->perm, rp->I64);
+            else oracle_force_matrix(n, prob->F, prob->D, rp->perm, rp->I);
             rp->E = oracle_energy(n, prob->F, prob->D, rp->perm);
             rp->bestE = INT64_MAX; /* CPU_PIM.cpp:123 (INT_MAX) */
             rp->pos = l;           /* CPU_PIM.cpp:115-116 */
@@ -568,6 +729,8 @@ int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_re
         L->beta2 = beta2;
         L->c20 = c20;
         L->dE = malloc(sizeof(int32_t) * n);
+        L->dE64 = malloc(sizeof(int64_t) * n);
+        L->wide = wide;
         L->w = malloc(sizeof(uint32_t) * n);
         L->os_work = malloc(sizeof(int32_t) * 4 * (size_t)n);
         L->os_owner = malloc(sizeof(uint64_t) * (size_t)n);
@@ -654,6 +817,7 @@ int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_re
         res->exchanges += lad[t].exchanges;
         res->kicks += lad[t].kicks;
         free(lad[t].dE);
+        free(lad[t].dE64);
         free(lad[t].w);
         free(lad[t].os_work);
         free(lad[t].os_owner);
@@ -667,6 +831,7 @@ int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_re
         free(reps[r].pinv);
         free(reps[r].best_perm);
         free(reps[r].I);
+        free(reps[r].I64);
     }
     free(reps);
     free(lad);

@@ -90,6 +90,8 @@ void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t
 
 /* Integer heat-bath weight floor(2^20 * 2^(beta2f*d)) for d <= 0, own exp2. */
 uint32_t oracle_weight(float beta2f, int32_t d);
+/* the same weight for a 64-bit difference d (instances beyond the int32 bounds) */
+uint32_t oracle_weight64(float beta2f, int64_t d);
 
 /* I[i][a] = -sum_j (D[i][j] F[a][p(j)] + D[j][i] F[p(j)][a])   (CPU_PIM.cpp:164-181) */
 void oracle_force_matrix(int32_t n, const int32_t *F, const int32_t *D, const int32_t *perm,
@@ -143,7 +145,11 @@ double oracle_beta2(double beta_ln);
 void oracle_init_perm(int32_t n, uint64_t seed, uint32_t stream, int32_t *perm);
 
 /* Full solve.  nthreads > 1 runs ladders on POSIX threads (only without a goal,
- * where ladders never interact).  Returns 0 or a negative error. */
+ * where ladders never interact).  Returns 0 or a negative error (-4: the entries are
+ * too large even for 64-bit force matrices, or too large for one-shot steps).
+ * Force matrices are int32 when the permutation-independent bounds allow it
+ * (4 B + 16 maxD maxF < 2^30, B the force bound), else int64 with int64 deltas
+ * (the wide path: 4 B + 16 maxD maxF < 2^62 and n^2 maxD maxF < 2^62). */
 int oracle_solve(const oracle_problem *prob, const oracle_params *par, oracle_result *res,
                  int32_t mode, int32_t nthreads);
 

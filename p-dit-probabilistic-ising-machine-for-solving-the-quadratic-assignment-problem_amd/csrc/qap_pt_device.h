@@ -48,9 +48,8 @@ __host__ __device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
 // The maximum candidate gets exactly 2^20; the sum over n <= 1024 candidates stays
 // below 2^31.
 // ---------------------------------------------------------------------------
-__host__ __device__ inline uint32_t heat_weight(float beta2f, int32_t d) {
+__host__ __device__ inline uint32_t heat_weight_y(float y) {
 #pragma clang fp contract(off)
-    const float y = beta2f * (float)d;
     if (!(y > -32.0f)) return 0u;
     const float kf = rintf(y);
     const float f = y - kf;
@@ -67,11 +66,30 @@ __host__ __device__ inline uint32_t heat_weight(float beta2f, int32_t d) {
     return (uint32_t)(p * (float)(1u << e));
 }
 
+__host__ __device__ inline uint32_t heat_weight(float beta2f, int32_t d) {
+#pragma clang fp contract(off)
+    return heat_weight_y(beta2f * (float)d);
+}
+
+// The same weight for a 64-bit difference (force matrices beyond int32): (float)d is the
+// correctly rounded conversion, and the float steps from y on are the same.
+__host__ __device__ inline uint32_t heat_weight64(float beta2f, int64_t d) {
+#pragma clang fp contract(off)
+    return heat_weight_y(beta2f * (float)d);
+}
+
 // CPU_PIM.cpp:234-235: cutoff = (int)(maxDeltaE - 20.0 / beta), saturated below.
 __host__ __device__ inline int32_t heat_cutoff(int32_t maxd, double c20) {
     const double cd = (double)maxd - c20;
     if (cd < -2147483648.0) return INT32_MIN;
     return (int32_t)cd;
+}
+
+// The same cutoff in 64 bits (force matrices beyond int32).
+__host__ __device__ inline int64_t heat_cutoff64(int64_t maxd, double c20) {
+    const double cd = (double)maxd - c20;
+    if (cd < -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)cd;
 }
 
 // ---------------------------------------------------------------------------
@@ -92,6 +110,7 @@ struct InstDesc {
     int64_t vec_off;          // Fdiag/Ddiag element offset (n)
     int64_t p_base;           // perm/pinv/best_perm element offset of replica 0
     int64_t I_base;           // force-matrix element offset of replica 0
+    int64_t I64_base;         // int64 force matrices (instances beyond int32), replica 0
     int32_t rep_base;         // first global replica index (also order[] offset)
     int32_t ladder_base;      // first global ladder index
     int32_t beta_off;         // per-position tables offset
@@ -123,6 +142,7 @@ struct PoolDev {
     const float *beta2f;      // per ladder position (exp2 units, float)
     const double *c20;        // per ladder position: 20 / beta2 (double)
     int32_t *perm, *pinv, *best_perm, *I;
+    int64_t *I64;             // force matrices of instances whose bounds exceed int32
     int64_t *E, *bestE;
     uint64_t *accepted;
     uint64_t *cells;          // force-matrix entries rewritten by rank-2 updates

@@ -22,6 +22,8 @@ int pool_create_internal(int32_t count, const qap_problem *problems, const qap_p
 int pool_launch_async(qap_pt_pool *p, hipEvent_t done);
 int pool_result_internal(qap_pt_pool *p, int inst, qap_pt_result *res);
 void pool_destroy_internal(qap_pt_pool *p);
+// Whether the instance's bounds need int64 force matrices (the int64 in-HBM placement).
+bool wide_instance(const qap_problem &pb);
 // Device bytes a pool of these instances allocates.
 int64_t pool_device_bytes(int32_t count, const qap_problem *problems, const qap_pt_params *params);
 

@@ -40,6 +40,7 @@ enum IMode : int {
     IM_LAZY = 3,   // int32 in HBM, rank-2 updates held in LDS and applied in batches
                    // (symmetric instances; 16-bit update vectors)
     IM_ONESHOT = 4,  // one-shot parallel-proposal steps, int32 in HBM (QAP_PT_ONE_SHOT)
+    IM_HBM64 = 5,    // int64 in HBM, updated in place (bounds beyond int32)
 };
 
 struct SweepShape {
@@ -74,6 +75,13 @@ hipError_t launch_sweep_persistent(const PoolDev &P, const SweepShape &sh, const
                                    int nblocks, int lds_bytes, int intervals, hipStream_t s);
 // whether a persistent launch of nblocks workgroups of this shape fits the device at once
 bool sweep_persistent_fits(const SweepShape &sh, int nblocks, int lds_bytes);
+hipError_t launch_sweep64(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,
+                          int lds_bytes, hipStream_t s);
+int sweep64_lds_bytes(int n);
+hipError_t launch_force64(const int32_t *F, const int32_t *D, const int32_t *perm, int n, int ld,
+                          int nrep, int64_t *Iout, hipStream_t s);
+hipError_t launch_count_mismatch64(const int64_t *a, const int64_t *b, int n, int ld,
+                                   int64_t total_rows, unsigned long long *out, hipStream_t s);
 hipError_t launch_sweep_lazy(const PoolDev &P, const SweepShape &sh, const int32_t *list,
                              int nblocks, int lds_bytes, hipStream_t s);
 hipError_t launch_oneshot(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,

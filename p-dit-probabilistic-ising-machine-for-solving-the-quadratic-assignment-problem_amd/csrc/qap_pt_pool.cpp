@@ -80,12 +80,27 @@ Bounds compute_bounds(int n, const int32_t *F, const int32_t *D) {
         sD = std::max(sD, rd);
         sF = std::max(sF, rf);
     }
+    // products in 128 bits: with entries near 2^31 the bounds pass 2^63 (saturated at
+    // INT64_MAX; such an instance is refused)
+    const __int128 force = std::min((__int128)maxF * sD, (__int128)maxD * sF);
+    const __int128 delta = 4 * force + 16 * (__int128)maxD * maxF;
+    const __int128 top = INT64_MAX;
     Bounds b;
-    b.force = std::min(maxF * sD, maxD * sF);
-    b.delta = 4 * b.force + 16 * maxD * maxF;
+    b.force = (int64_t)std::min(force, top);
+    b.delta = (int64_t)std::min(delta, top);
     b.maxD = maxD;
     b.maxF = maxF;
     return b;
+}
+
+// 0: force matrices, deltas and update terms provably fit int32; 1: they fit int64 only
+// (4 B + 16 maxD maxF < 2^62 and the cost n^2 maxD maxF < 2^62); -1: neither.
+int width_class(int n, const Bounds &b) {
+    if (b.delta < ((int64_t)1 << 30) && b.force + 8 * b.maxD * b.maxF < ((int64_t)1 << 31)) return 0;
+    if (b.delta < ((int64_t)1 << 62) &&
+        (__int128)n * n * b.maxD * b.maxF < ((__int128)1 << 62))
+        return 1;
+    return -1;
 }
 
 bool is_sym_zero_diag(int n, const int32_t *M) {
@@ -238,6 +253,7 @@ const char *placement_name(int im) {
     case IM_LDS16: return "lds16";
     case IM_LAZY: return "lazy";
     case IM_ONESHOT: return "oneshot";
+    case IM_HBM64: return "hbm64";
     default: return "hbm";
     }
 }
@@ -304,11 +320,16 @@ int validate(const qap_problem &pb, const qap_pt_params &pr, int k) {
             return fail(QAP_PT_E_INVAL,
                         "instance %d: one-shot steps need 2 max|D| and 4 max|F| <= 32639", k);
     }
-    if (b.delta >= ((int64_t)1 << 30) || b.force + 8 * b.maxD * b.maxF >= ((int64_t)1 << 31))
+    const int width = width_class(pb.n, b);
+    if (width < 0)
         return fail(QAP_PT_E_OVERFLOW,
-                    "instance %d: |force| bound %lld / |delta| bound %lld exceed the int32 range "
+                    "instance %d: |force| bound %lld / |delta| bound %lld exceed the int64 range "
                     "the sweep kernels prove exact",
                     k, (long long)b.force, (long long)b.delta);
+    if (width > 0 && (pr.flags & QAP_PT_ONE_SHOT))
+        return fail(QAP_PT_E_OVERFLOW,
+                    "instance %d: one-shot steps need force bounds within int32 (|delta| bound "
+                    "%lld)", k, (long long)b.delta);
     return QAP_PT_OK;
 }
 
@@ -329,7 +350,7 @@ void pool_free(qap_pt_pool *p) {
         if (b.d_list) (void)hipFree(b.d_list);
     void *bufs[] = {p->d_inst,  p->d_state,     p->d_rep_inst, p->d_F,      p->d_D, p->d_fclass, p->d_fspan,
                     p->d_Ft,    p->d_Dt,        p->d_Fdiag,    p->d_Ddiag,  p->d_beta2f,
-                    p->d_c20,   p->d_perm,      p->d_pinv,     p->d_best_perm, p->d_I,
+                    p->d_c20,   p->d_perm,      p->d_pinv,     p->d_best_perm, p->d_I, p->d_I64,
                     p->d_E,     p->d_bestE,     p->d_limits,   p->d_accepted, p->d_cells, p->d_pos,
                     p->d_order, p->d_tss,       p->d_ladder_best, p->d_sync};
     for (void *b : bufs)
@@ -514,6 +535,12 @@ void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throu
                 h.shape.cpl = cpl;
             }
         }
+    } else if (width_class(n, bnd) > 0) {
+        // beyond int32: int64 force matrix in HBM, updated in place (sizes by the size rule)
+        h.shape = SweepShape{};
+        h.shape.im = IM_HBM64;
+        h.shape.sym = sym;
+        shape_for_size(n, sparse, h.shape);
     } else {
         h.shape = choose_shape(n, sym, nonneg, bnd.force, sparse, small_uv, throughput, &ld_s);
     }
@@ -620,6 +647,7 @@ void bind_pool(qap_pt_pool *p) {
     P.pinv = p->d_pinv;
     P.best_perm = p->d_best_perm;
     P.I = p->d_I;
+    P.I64 = p->d_I64;
     P.E = p->d_E;
     P.bestE = p->d_bestE;
     P.accepted = p->d_accepted;
@@ -694,7 +722,12 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
         p->total_mat += (int64_t)n * n;
         p->total_vec += n;
         p->total_p += (int64_t)RT * n;
-        p->total_I += (int64_t)RT * n * d.ld;
+        if (h.shape.im == IM_HBM64) {  // int64 force matrices live apart
+            d.I64_base = p->total_I64;
+            p->total_I64 += (int64_t)RT * n * d.ld;
+        } else {
+            p->total_I += (int64_t)RT * n * d.ld;
+        }
         p->total_reps += RT;
         p->total_ladders += d.T;
         p->total_beta += d.R;
@@ -777,6 +810,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(palloc(p, &p->d_pinv, p->total_p));
     POOL_TRY(palloc(p, &p->d_best_perm, p->total_p));
     POOL_TRY(palloc(p, &p->d_I, p->total_I));
+    POOL_TRY(palloc(p, &p->d_I64, p->total_I64));
     POOL_TRY(palloc(p, &p->d_E, RTall));
     POOL_TRY(palloc(p, &p->d_bestE, RTall));
     POOL_TRY(palloc(p, &p->d_accepted, RTall));
@@ -832,6 +866,8 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(up(p->d_ladder_best, ladder_best.data(), 8 * ladder_best.size()));
     POOL_TRY(hipMemsetAsync(p->d_I, 0, sizeof(int32_t) * std::max<int64_t>(p->total_I, 1),
                             p->stream));
+    POOL_TRY(hipMemsetAsync(p->d_I64, 0, sizeof(int64_t) * std::max<int64_t>(p->total_I64, 1),
+                            p->stream));
     POOL_TRY(hipMemsetAsync(p->d_sync, 0, sizeof(uint32_t) * (1 + 2 * (size_t)count), p->stream));
 
     bind_pool(p);
@@ -840,8 +876,12 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     for (int k = 0; k < count; ++k) {
         const InstDesc &d = p->desc[k];
         const int RT = d.R * d.T;
-        POOL_TRY(launch_force(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n,
-                              d.ld, RT, p->d_I + d.I_base, p->stream));
+        if (p->shapes[k].im == IM_HBM64)
+            POOL_TRY(launch_force64(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base,
+                                    d.n, d.ld, RT, p->d_I64 + d.I64_base, p->stream));
+        else
+            POOL_TRY(launch_force(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base,
+                                  d.n, d.ld, RT, p->d_I + d.I_base, p->stream));
         POOL_TRY(launch_energy(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n,
                                RT, p->d_E + d.rep_base, p->stream));
     }
@@ -1083,13 +1123,17 @@ int pool_create_in_arena(int32_t count, const qap_problem *problems, const qap_p
     }
 }
 
+bool wide_instance(const qap_problem &pb) {
+    return width_class(pb.n, compute_bounds(pb.n, pb.F, pb.D)) > 0;
+}
+
 int64_t pool_device_bytes(int32_t count, const qap_problem *problems, const qap_pt_params *params) {
     int64_t bytes = 0;
     for (int k = 0; k < count; ++k) {
         const int64_t n = problems[k].n, RT = (int64_t)params[k].R * params[k].T;
         const int64_t ld = hbm_stride((int)n);
         bytes += 4 * (4 * n * n + 4 * n);            // F, D, Ft, Dt, diagonals, classes, spans
-        bytes += 4 * RT * n * ld;                     // force matrices
+        bytes += (wide_instance(problems[k]) ? 8 : 4) * RT * n * ld;  // force matrices
         bytes += 4 * 3 * RT * n;                      // perm, pinv, best_perm
         bytes += RT * (8 * 4 + 4 * 3);                // E, bestE, accepted, cells, pos, order, rep_inst
         bytes += 24 * params[k].T + 16 * params[k].R + sizeof(InstDesc) + sizeof(InstState) + 8;
@@ -1189,7 +1233,8 @@ int qap_pt_pool_verify(qap_pt_pool *p, int64_t *force_mismatches, int64_t *cost_
         int32_t *scratch = nullptr;
         int64_t *Es = nullptr;
         unsigned long long *cnt = nullptr;
-        const size_t words = (size_t)RT * d.n * d.ld;
+        const bool wide = p->shapes[k].im == IM_HBM64;
+        const size_t words = (size_t)RT * d.n * d.ld * (wide ? 2 : 1);
         if (hipMalloc((void **)&scratch, 4 * words) != hipSuccess ||
             hipMalloc((void **)&Es, 8 * RT) != hipSuccess ||
             hipMalloc((void **)&cnt, 8) != hipSuccess) {
@@ -1199,12 +1244,20 @@ int qap_pt_pool_verify(qap_pt_pool *p, int64_t *force_mismatches, int64_t *cost_
         }
         (void)hipMemsetAsync(scratch, 0, 4 * words, p->stream);
         (void)hipMemsetAsync(cnt, 0, 8, p->stream);
-        (void)launch_force(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n, d.ld, RT,
-                     scratch, p->stream);
+        if (wide) {
+            int64_t *s64 = reinterpret_cast<int64_t *>(scratch);
+            (void)launch_force64(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n,
+                                 d.ld, RT, s64, p->stream);
+            (void)launch_count_mismatch64(s64, p->d_I64 + d.I64_base, d.n, d.ld, (int64_t)RT * d.n,
+                                          cnt, p->stream);
+        } else {
+            (void)launch_force(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n,
+                               d.ld, RT, scratch, p->stream);
+            (void)launch_count_mismatch(scratch, p->d_I + d.I_base, d.n, d.ld, (int64_t)RT * d.n,
+                                        cnt, p->stream);
+        }
         (void)launch_energy(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n, RT, Es,
                       p->stream);
-        (void)launch_count_mismatch(scratch, p->d_I + d.I_base, d.n, d.ld, (int64_t)RT * d.n, cnt,
-                              p->stream);
         unsigned long long c = 0;
         std::vector<int64_t> e1(RT), e2(RT);
         hipError_t e = hipStreamSynchronize(p->stream);

@@ -42,7 +42,7 @@ struct qap_pt_pool {
     std::vector<qap::Bucket> buckets;
     std::vector<int64_t> limits;  // mega-iteration limit per instance (host mirror)
     int total_reps = 0, total_ladders = 0;
-    int64_t total_p = 0, total_I = 0, total_mat = 0, total_vec = 0, total_beta = 0;
+    int64_t total_p = 0, total_I = 0, total_I64 = 0, total_mat = 0, total_vec = 0, total_beta = 0;
     qap::PoolDev P{};
     // owned device buffers
     qap::InstDesc *d_inst = nullptr;
@@ -52,6 +52,7 @@ struct qap_pt_pool {
     float *d_beta2f = nullptr;
     double *d_c20 = nullptr;
     int32_t *d_perm = nullptr, *d_pinv = nullptr, *d_best_perm = nullptr, *d_I = nullptr;
+    int64_t *d_I64 = nullptr;  // int64 force matrices of instances beyond the int32 bounds
     int64_t *d_E = nullptr, *d_bestE = nullptr, *d_limits = nullptr;
     uint64_t *d_accepted = nullptr, *d_cells = nullptr;
     int32_t *d_pos = nullptr, *d_order = nullptr, *d_tss = nullptr;

@@ -44,6 +44,7 @@ struct Request {
     int32_t status = QAP_SRV_QUEUED;
     qap_pt_result res{};
     int64_t bytes = 0;
+    bool wide = false;  // needs int64 force matrices: runs in a pool of its own
 };
 
 struct Flight {
@@ -156,7 +157,7 @@ void bank_class(const Request &r, int &n_cap, int &rt_cap) {
     n_cap = 0;
     rt_cap = 0;
     const int64_t RT = (int64_t)r.par.R * r.par.T;
-    if (RT > 64) return;
+    if (RT > 64 || r.wide) return;  // banks hold int32 force matrices
     rt_cap = RT <= 32 ? 32 : 64;
     for (int c : kBankClasses)
         if (r.n <= c) {
@@ -480,6 +481,7 @@ int qap_srv_submit(qap_srv *s, const qap_problem *prob, const qap_pt_params *par
         r.par.init_perm = r.init.empty() ? nullptr : r.init.data();
         qap_problem pb{r.n, r.F.data(), r.D.data()};
         r.bytes = qap::pool_device_bytes(1, &pb, &r.par);
+        r.wide = qap::wide_instance(pb);
         if (s->cfg.device_budget > 0 && r.bytes > s->cfg.device_budget)
             return qap::fail(QAP_PT_E_NOMEM, "request needs %lld device bytes, budget is %lld",
                              (long long)r.bytes, (long long)s->cfg.device_budget);

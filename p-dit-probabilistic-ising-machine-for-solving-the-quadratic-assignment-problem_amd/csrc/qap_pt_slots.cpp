@@ -138,9 +138,11 @@ int slots_create(int32_t slots, int32_t n_cap, int32_t r_cap, int32_t t_cap, int
     return QAP_PT_OK;
 }
 
+// (slots hold int32 force matrices: an instance beyond the int32 bounds runs in a pool of
+// its own)
 bool slots_fit(const qap_pt_pool *p, const qap_problem &pb, const qap_pt_params &pr) {
     return p && p->slotted && pb.n <= p->n_cap && pr.R <= p->r_cap && pr.T <= p->t_cap &&
-           (int64_t)pr.R * pr.T <= p->rt_cap;
+           (int64_t)pr.R * pr.T <= p->rt_cap && !wide_instance(pb);
 }
 
 int slots_free(const qap_pt_pool *p) {

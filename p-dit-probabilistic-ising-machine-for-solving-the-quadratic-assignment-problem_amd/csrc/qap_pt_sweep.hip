@@ -1201,6 +1201,7 @@ hipError_t launch_sweep(const PoolDev &P, const SweepShape &sh, const int32_t *l
     if (nblocks <= 0) return hipSuccess;
     if (sh.im == IM_LAZY) return launch_sweep_lazy(P, sh, list, nblocks, lds_bytes, s);
     if (sh.im == IM_ONESHOT) return launch_oneshot(P, sh, list, nblocks, lds_bytes, s);
+    if (sh.im == IM_HBM64) return launch_sweep64(P, sh, list, nblocks, lds_bytes, s);
     return launch_sweep_iv(P, sh, list, nblocks, lds_bytes, 1, s);
 }
 
@@ -1218,6 +1219,8 @@ bool sweep_persistent_fits(const SweepShape &sh, int nblocks, int lds_bytes) {
 
 bool sweep_shape_supported(int nt, int cpl, int im) {
     if (im == IM_ONESHOT) return (nt == 256 || nt == 512) && (cpl == 1 || cpl == 2 || cpl == 4);
+    if (im == IM_HBM64)
+        return (cpl == 1 && (nt == 64 || nt == 128 || nt == 256)) || (nt == 256 && (cpl == 2 || cpl == 4));
     if (im == IM_LAZY && nt == 512 && (cpl == 1 || cpl == 2)) return true;
     switch (nt * 8 + cpl) {
     case 64 * 8 + 1: case 64 * 8 + 2: case 64 * 8 + 4: case 128 * 8 + 1: case 128 * 8 + 2:
@@ -1228,6 +1231,7 @@ bool sweep_shape_supported(int nt, int cpl, int im) {
 
 int sweep_lds_bytes(int n, int ld_s, int im) {
     if (im == IM_ONESHOT) return oneshot_lds_bytes(n);
+    if (im == IM_HBM64) return sweep64_lds_bytes(n);
     return im == IM_LAZY ? lazy_lds_bytes(n) : 4 * sweep_lds_words(n, ld_s, im);
 }
 

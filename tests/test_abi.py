@@ -47,9 +47,18 @@ def test_host_validation_errors(qpt):
     _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(**dict(ok, beta_ln=np.array([0.0, 0.1]))))
     _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(**dict(ok, sweeps=5, sweeps_per_exchange=2)))
     _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(**dict(ok, init_perm=np.zeros(8, np.int32))))
-    big = np.full((8, 8), 1 << 20, dtype=np.int32)
+    # entries of 2^30: the deltas exceed even the int64 force matrices' bound
+    big = np.full((8, 8), 1 << 30, dtype=np.int32)
     np.fill_diagonal(big, 0)
     _expect(qpt, qpt.QAP_PT_E_OVERFLOW, qpt.Instance(**dict(ok, F=big, D=big)))
+    # entries of 2^20 pass the host checks (int64 force matrices); without a device the
+    # pool then fails on the device, not on the bounds
+    wide = np.full((8, 8), 1 << 20, dtype=np.int32)
+    np.fill_diagonal(wide, 0)
+    if qpt.device_count() < 1:
+        _expect(qpt, qpt.QAP_PT_E_DEVICE, qpt.Instance(**dict(ok, F=wide, D=wide)))
+    else:
+        qpt.Pool([qpt.Instance(**dict(ok, F=wide, D=wide))]).close()
     Fz = np.zeros((QMAX := qpt.MAX_N + 1, QMAX), np.int32)
     _expect(qpt, qpt.QAP_PT_E_INVAL, qpt.Instance(**dict(ok, F=Fz, D=Fz)))
 

@@ -106,6 +106,26 @@ def _twins(seed, n, blocks=((0, 8, 4), (8, 16, 9), (16, 24, 0)), pairs=((30, 31,
     return F, (D + D.T).astype(np.int32)
 
 
+def _wide64(seed, n, symmetric=True, zero_diag=True, big=1 << 27):
+    """Flow entries up to 2^27 and distances up to 63: force bounds beyond int32, so the
+    instance runs with int64 force matrices; costs reach ~1e13."""
+    g = np.random.default_rng(seed)
+    F = g.integers(0, big, (n, n), dtype=np.int64)
+    D = g.integers(0, 64, (n, n), dtype=np.int64)
+    if symmetric:
+        F = np.triu(F, 1) + np.triu(F, 1).T
+        D = np.triu(D, 1) + np.triu(D, 1).T
+    if zero_diag:
+        np.fill_diagonal(F, 0)
+        np.fill_diagonal(D, 0)
+    return F.astype(np.int32), D.astype(np.int32)
+
+
+def _wide_scale(n):
+    """beta scale that puts the int64 instances' deltas (~4 * 63 * 2^27 * n) at ~20 / beta"""
+    return 20.0 / (4 * 63 * (1 << 27) * n) * max(1.0, n / 16.0)
+
+
 CASES = {
     # name: (instance builder, R, sweeps, extra kwargs)
     "sym_n16_R4": (lambda q: q.instances.random_instance(11, 16), 4, 40, {}),
@@ -218,6 +238,19 @@ CASES = {
                            3, 4, {"one_shot": True, "env": {"QAP_PT_SHAPE": "256,2"}}),
     "os_n50_256x1": (lambda q: q.instances.random_instance(39, 50), 3, 6,
                      {"one_shot": True, "env": {"QAP_PT_SHAPE": "256,1"}}),
+    # entries beyond the int32 force bound: int64 force matrices in HBM (hbm64 placement)
+    "wide64_sym_n40": (lambda q: _wide64(80, 40), 4, 10,
+                       {"beta_scale": _wide_scale(40), "expect_shape": "64x1,hbm64"}),
+    "wide64_asym_diag_n37_T2": (lambda q: _wide64(81, 37, symmetric=False, zero_diag=False), 3, 8,
+                                {"T": 2, "beta_scale": _wide_scale(37), "kick_after": 2,
+                                 "expect_shape": "64x1,hbm64,general"}),
+    "wide64_n100_128x1": (lambda q: _wide64(82, 100), 3, 4,
+                          {"beta_scale": _wide_scale(100), "expect_shape": "128x1,hbm64"}),
+    "wide64_n300_256x2": (lambda q: _wide64(83, 300), 2, 1,
+                          {"beta_scale": _wide_scale(300), "expect_shape": "256x2,hbm64"}),
+    "wide64_n130_256x1_spe2": (lambda q: _wide64(84, 130, symmetric=False), 2, 4,
+                               {"spe": 2, "beta_scale": _wide_scale(130),
+                                "expect_shape": "256x1,hbm64,general"}),
     # the benchmark configurations at their own sizes, shapes and ladders
     # n = 1024 large dense: lazy placement, eight waves x 2 candidates, 24 pending updates
     "lazy_c5_n1024_R2": (lambda q: q.instances.large_c5(1024), 2, 1,
@@ -442,3 +475,29 @@ def test_throughput_pool_more_replicas_than_cus(qpt, oracle, gpu):
         ref = oracle.solve(inst.F, inst.D, inst.beta_ln, inst.sweeps, seed=inst.seed)
         print(f"instance {k}: n={inst.n} shape={shapes[k]}", flush=True)
         _assert_same(got[k][0], got[k][1], (fm, cm), ref)
+
+
+def test_wide_instance_through_server_and_noalloc(qpt, oracle, gpu):
+    """An instance beyond the int32 bounds through the serving loop (slot banks hold int32
+    matrices, so it runs in a pool of its own beside banked requests) and through the
+    allocation-free solve: both equal the allocating solve and the oracle."""
+    F, D = _wide64(90, 48, symmetric=False, zero_diag=False)
+    betas = qpt.instances.geometric_ladder(4, 1.0, 100.0) / 3.0 * _wide_scale(48)
+    wide = qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=6, seed=31)
+    Fs, Ds = qpt.instances.mixed_instance(91, 40, dense=True)
+    small = qpt.Instance(F=Fs, D=Ds, beta_ln=qpt.instances.geometric_ladder(4, 1.0, 100.0) / 2.5,
+                         sweeps=6, seed=32)
+    want = [qpt.solve(wide), qpt.solve(small)]
+    ref = oracle.solve(F, D, betas, 6, seed=31)
+    assert want[0]["best_cost"] == ref["best_cost"]
+    np.testing.assert_array_equal(want[0]["best_perm"], ref["best_perm"])
+    with qpt.Server(max_pools=4, max_batch=4) as srv:
+        tickets = [srv.submit(wide), srv.submit(small)]
+        got = [srv.wait(t, timeout_s=60.0) for t in tickets]
+    for g, w in zip(got, want):
+        assert g["status"] == qpt.QAP_SRV_DONE
+        assert g["best_cost"] == w["best_cost"] and g["accepted"] == w["accepted"]
+        np.testing.assert_array_equal(g["best_perm"], w["best_perm"])
+    na = qpt.solve_noalloc(wide)
+    assert na["best_cost"] == want[0]["best_cost"] and na["accepted"] == want[0]["accepted"]
+    np.testing.assert_array_equal(na["best_perm"], want[0]["best_perm"])
