@@ -82,6 +82,28 @@ hipError_t launch_force64(const int32_t *F, const int32_t *D, const int32_t *per
                           int nrep, int64_t *Iout, hipStream_t s);
 hipError_t launch_count_mismatch64(const int64_t *a, const int64_t *b, int n, int ld,
                                    int64_t total_rows, unsigned long long *out, hipStream_t s);
+// Flush helpers of the lazy kernel: with `helpers` > 0 the launch has nrep replica
+// workgroups and helpers x nrep helper workgroups (cooperative, all resident); a replica's
+// flush is split by rows between it and its helpers, which read the pending updates from
+// `uv` (per replica block: U then V, uv_stride int16 elements) and synchronise through
+// `words`: [0] fault (a wait that gave up), then per replica block r [1 + 3 r] flush
+// requests, [2 + 3 r] pending count, [3 + 3 r] helper-wave arrivals.  Words after [0] must
+// be zero at the start of every launch.
+struct FlushHelp {
+    int16_t *uv = nullptr;
+    uint32_t *words = nullptr;
+    int64_t uv_stride = 0;
+    int nrep = 0;
+    int helpers = 0;
+};
+// int16 elements of pending updates (U and V) a lazy workgroup of size n holds with
+// lds_bytes of LDS
+int64_t lazy_pending_elems(int n, int lds_bytes);
+// Whether nblocks x (1 + helpers) workgroups of this lazy shape are resident at once.
+bool lazy_helpers_fit(const SweepShape &sh, int nblocks, int helpers, int lds_bytes);
+// A lazy launch with flush helpers (cooperative); grid = fh.nrep * (1 + fh.helpers).
+hipError_t launch_sweep_lazy_helped(const PoolDev &P, const SweepShape &sh, const int32_t *list,
+                                    int lds_bytes, const FlushHelp &fh, hipStream_t s);
 hipError_t launch_sweep_lazy(const PoolDev &P, const SweepShape &sh, const int32_t *list,
                              int nblocks, int lds_bytes, hipStream_t s);
 hipError_t launch_oneshot(const PoolDev &P, const SweepShape &sh, const int32_t *list, int nblocks,

@@ -48,10 +48,14 @@ __device__ __forceinline__ void split_bytes(v4h x, long &lo, long &hi) {
 // l % 16).  Row tiles are loaded DEPTH - 1 tiles ahead of their update; the loads
 // are unconditional (lanes past the matrix read its last row / 4 columns and store
 // nothing), so no branch keeps them from staying in flight.
+//
+// Rows [x_lo, x_hi) only (x_lo a multiple of 16): the flush of one matrix split among
+// several workgroups.  U and V may be in LDS or in global memory.
 template <int NW, int KCH, int DEPTH = kUpdDepth>
 __device__ __forceinline__ void mfma_rank_update(int32_t *Ig, int ld, int n, const int16_t *U_s,
                                                  const int16_t *V_s, int kst, int K, int lane,
-                                                 int wid) {
+                                                 int wid, int x_lo = 0, int x_hi = -1) {
+    if (x_hi < 0) x_hi = n;
     const int groups = K / 8 + ((K & 7) != 0);
     const int kch = (groups + 3) / 4;
     const int li = lane & 15, kg = lane >> 4;
@@ -84,7 +88,7 @@ __device__ __forceinline__ void mfma_rank_update(int32_t *Ig, int ld, int n, con
             }
         };
         auto update_rows = [&](const v4i(&src)[kUpdTiles], int x0) {
-            if (x0 >= n) return;
+            if (x0 >= x_hi) return;
             const int x = x0 + li;
             long b0[KCH], b1[KCH];
             bool uh[KCH];
@@ -92,7 +96,7 @@ __device__ __forceinline__ void mfma_rank_update(int32_t *Ig, int ld, int n, con
             for (int kc = 0; kc < KCH; ++kc) {
                 const int g = 4 * kc + kg;
                 v4h w = v4h{0, 0, 0, 0};
-                if (x < n && g < groups) w = reinterpret_cast<const v4h *>(U_s + (size_t)x * kst)[g];
+                if (x < x_hi && g < groups) w = reinterpret_cast<const v4h *>(U_s + (size_t)x * kst)[g];
                 split_bytes(w, b0[kc], b1[kc]);
                 uh[kc] = __ballot(b1[kc] != 0) != 0;  // some U entry needs its high byte
             }
@@ -118,17 +122,17 @@ __device__ __forceinline__ void mfma_rank_update(int32_t *Ig, int ld, int n, con
                 for (int v = 0; v < 4; ++v)
                     r[v] = (int32_t)((uint32_t)src[t][v] + (uint32_t)s0[v] + ((uint32_t)s1[v] << 8) +
                                      ((uint32_t)s2[v] << 16));
-                if (x < n && c < n) *reinterpret_cast<v4i *>(Ig + (int64_t)x * ld + c) = r;
+                if (x < x_hi && c < n) *reinterpret_cast<v4i *>(Ig + (int64_t)x * ld + c) = r;
             }
         };
         // scheduling barriers keep the loads in row-tile order, so waiting for the
         // oldest tile does not wait for the younger ones
 #pragma unroll
         for (int dd = 0; dd < DEPTH - 1; ++dd) {
-            load_rows(cur[dd], 16 * dd);
+            load_rows(cur[dd], x_lo + 16 * dd);
             __builtin_amdgcn_sched_barrier(0);
         }
-        for (int x0 = 0; x0 < n; x0 += 16 * DEPTH) {
+        for (int x0 = x_lo; x0 < x_hi; x0 += 16 * DEPTH) {
 #pragma unroll
             for (int dd = 0; dd < DEPTH; ++dd) {
                 load_rows(cur[(dd + DEPTH - 1) % DEPTH], x0 + 16 * (dd + DEPTH - 1));

@@ -352,7 +352,8 @@ void pool_free(qap_pt_pool *p) {
                     p->d_Ft,    p->d_Dt,        p->d_Fdiag,    p->d_Ddiag,  p->d_beta2f,
                     p->d_c20,   p->d_perm,      p->d_pinv,     p->d_best_perm, p->d_I, p->d_I64,
                     p->d_E,     p->d_bestE,     p->d_limits,   p->d_accepted, p->d_cells, p->d_pos,
-                    p->d_order, p->d_tss,       p->d_ladder_best, p->d_sync};
+                    p->d_order, p->d_tss,       p->d_ladder_best, p->d_sync, p->d_uv,
+                    p->d_fwords};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto st : p->side) (void)hipStreamDestroy(st);
@@ -468,7 +469,35 @@ hipError_t enqueue_persistent(qap_pt_pool *p, int intervals, const hipEvent_t *e
     return e;
 }
 
+// One exchange interval of a helped lazy bucket: exchange step, cleared flush words, the
+// cooperative sweep launch with its helpers (events around it when `ev`).
+hipError_t enqueue_helped(qap_pt_pool *p, const hipEvent_t *ev) {
+    const Bucket &B = p->buckets[0];
+    FlushHelp fh;
+    fh.uv = p->d_uv;
+    fh.words = p->d_fwords;
+    fh.uv_stride = p->uv_stride;
+    fh.nrep = (int)B.reps.size();
+    fh.helpers = p->helpers;
+    hipError_t e = launch_exchange(p->P, p->count, p->max_rt, p->stream);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(p->d_fwords + 1, 0, sizeof(uint32_t) * 3 * (size_t)fh.nrep, p->stream);
+    if (e == hipSuccess && ev) e = hipEventRecord(ev[0], p->stream);
+    if (e == hipSuccess) e = launch_sweep_lazy_helped(p->P, B.shape, B.d_list, B.lds_bytes, fh, p->stream);
+    if (e == hipSuccess && ev) e = hipEventRecord(ev[1], p->stream);
+    return e;
+}
+
 int enqueue_intervals(qap_pt_pool *p, int64_t intervals) {
+    if (p->helpers > 0) {
+        for (int64_t m = 0; m < intervals; ++m) {
+            const hipError_t e = enqueue_helped(p, nullptr);
+            if (e != hipSuccess)
+                return fail(QAP_PT_E_DEVICE, "helped lazy sweep launch failed: %s",
+                            hipGetErrorString(e));
+        }
+        return QAP_PT_OK;
+    }
     if (p->persistent) {
         for (int64_t m = 0; m < intervals; m += kPersistMaxIntervals) {
             const int n = (int)std::min<int64_t>(intervals - m, kPersistMaxIntervals);
@@ -777,6 +806,30 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
                         sweep_persistent_fits(p->buckets[0].shape, (int)p->buckets[0].reps.size(),
                                               p->buckets[0].lds_bytes);
     }
+    // Flush helpers: a latency-bound pool whose replicas all run the lazy kernel leaves
+    // CUs idle; helper workgroups can take shares of every flush.  Handing rows between
+    // workgroups costs agent-scope fences, which write back or invalidate a whole XCD's L2
+    // (shared with the other replicas there), so they pay only where flushes are largest:
+    // one helper per replica from n = 768 (n = 1024: 13.6 -> 12.3 ms per sweep launch;
+    // n = 512 lost 17 % with one helper and 37 % with three).  QAP_PT_FLUSH_HELPERS=k sets
+    // k (0..3) for any lazy latency-bound pool.
+    if (!arena && !throughput && p->buckets.size() == 1 && p->buckets[0].shape.im == IM_LAZY) {
+        const Bucket &B = p->buckets[0];
+        const int nb = (int)B.reps.size();
+        int n_max = 0;
+        for (int k = 0; k < count; ++k) n_max = std::max(n_max, p->desc[k].n);
+        int h = n_max >= 768 ? 1 : 0;
+        if (const char *env = std::getenv("QAP_PT_FLUSH_HELPERS")) h = std::max(0, std::atoi(env));
+        h = std::min({h, 3, device_cus() / std::max(nb, 1) - 1});
+        while (h > 0 && !lazy_helpers_fit(B.shape, nb, h, B.lds_bytes)) --h;
+        if (h > 0) {
+            int64_t stride = 0;
+            for (int k = 0; k < count; ++k)
+                stride = std::max(stride, lazy_pending_elems(p->desc[k].n, B.lds_bytes));
+            p->uv_stride = (stride + 7) & ~(int64_t)7;  // 16-byte aligned per replica
+            p->helpers = h;
+        }
+    }
 
     auto bail = [&](int rc) {
         pool_free(p);
@@ -821,6 +874,12 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(palloc(p, &p->d_ladder_best, p->total_ladders));
     POOL_TRY(palloc(p, &p->d_sync, 1 + 2 * (size_t)count));
     for (auto &b : p->buckets) POOL_TRY(palloc(p, &b.d_list, b.reps.size()));
+    if (p->helpers > 0) {  // pending updates and flush words of the helped lazy bucket
+        const size_t nb = p->buckets[0].reps.size();
+        POOL_TRY(dalloc(&p->d_uv, nb * (size_t)p->uv_stride));
+        POOL_TRY(dalloc(&p->d_fwords, 1 + 3 * nb));
+        POOL_TRY(hipMemset(p->d_fwords, 0, sizeof(uint32_t) * (1 + 3 * nb)));
+    }
     if (p->arena && p->arena_used > p->arena_size)
         return bail(fail(QAP_PT_E_NOMEM,
                          "the instances need %zu bytes of device memory; the fixed scratch "
@@ -958,7 +1017,8 @@ int pool_run_impl(qap_pt_pool *p, int64_t sweeps, double *kernel_ms, int64_t *la
                 m += n;
             }
             for (; !p->persistent && used < chunk && m < max_new; ++used, ++m) {
-                const hipError_t e = enqueue_interval(p, ev.data() + 2 * used * nb);
+                const hipError_t e = p->helpers > 0 ? enqueue_helped(p, ev.data() + 2 * used * nb)
+                                                    : enqueue_interval(p, ev.data() + 2 * used * nb);
                 if (e != hipSuccess) {
                     rc = fail(QAP_PT_E_DEVICE, "interval launch failed: %s", hipGetErrorString(e));
                     break;
@@ -1014,12 +1074,13 @@ int pool_run_impl(qap_pt_pool *p, int64_t sweeps, double *kernel_ms, int64_t *la
     }
     HIP_TRY(hipStreamSynchronize(p->stream));
     p->seconds += elapsed();
-    if (p->persistent) {
+    if (p->persistent || p->helpers > 0) {
         uint32_t fault = 0;
-        HIP_TRY(hipMemcpy(&fault, p->d_sync, sizeof(fault), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&fault, p->persistent ? p->d_sync : p->d_fwords, sizeof(fault),
+                          hipMemcpyDeviceToHost));
         if (fault)
-            return fail(QAP_PT_E_DEVICE, "a persistent sweep launch gave up waiting for the "
-                                         "replicas of its ladder");
+            return fail(QAP_PT_E_DEVICE, "a cooperative sweep launch gave up waiting for the "
+                                         "other workgroups of its replica or ladder");
     }
     return QAP_PT_OK;
 }

@@ -60,6 +60,12 @@ struct qap_pt_pool {
     // latency-bound pool of LDS placements: exchange intervals run as persistent
     // cooperative launches (replicas stay in LDS, exchange steps in the kernel)
     bool persistent = false;
+    // latency-bound pool of one lazy bucket: every sweep launch brings `helpers` flush
+    // helper workgroups per replica (cooperative launches, no graphs)
+    int helpers = 0;
+    int16_t *d_uv = nullptr;       // pending updates handed to the helpers
+    uint32_t *d_fwords = nullptr;  // fault word + 3 words per replica block
+    int64_t uv_stride = 0;
     int64_t *d_ladder_best = nullptr;
     std::vector<hipStream_t> side;     // one per extra bucket (default priority)
     hipEvent_t ev_fork = nullptr;

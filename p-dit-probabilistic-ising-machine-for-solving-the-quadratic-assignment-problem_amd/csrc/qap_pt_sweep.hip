@@ -413,7 +413,9 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
             }
         } else if (tid == 0) {
             const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-            while (__hip_atomic_load(&P.sync[2 + 2 * inst], __ATOMIC_ACQUIRE,
+            // relaxed polls (an acquire per poll would invalidate the caches other
+            // workgroups read from); the acquire fence after the barrier below follows
+            while (__hip_atomic_load(&P.sync[2 + 2 * inst], __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT) != target) {
                 __builtin_amdgcn_s_sleep(2);
                 if (__builtin_amdgcn_s_memrealtime() - t0 > kPersistWaitTicks) {

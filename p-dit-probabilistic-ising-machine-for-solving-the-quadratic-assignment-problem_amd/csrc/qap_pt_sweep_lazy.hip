@@ -88,15 +88,33 @@ bool lazy_fits(int n) { return lazy_layout(n, lazy_lds_bytes(n)).kmax >= 8; }
 template <int NT> constexpr int kLazyWaves = NT >= 512 ? 2 : 3;
 template <int NT> constexpr int kLazyFlushDepth = NT >= 512 ? kUpdDepth : 2;
 
+// Longest a flush helper or a replica waits at a flush: 30 s of the 100 MHz real-time
+// counter (the launch is cooperative, so the waits end; the limit turns a fault into a
+// reported error instead of a hang).
+constexpr uint64_t kFlushWaitTicks = 3000000000ull;
+constexpr uint32_t kFlushStop = 0xffffffffu;
+
+// Rows [lo, hi) of part p of a flush split into `parts` runs of whole 16-row tiles.
+__device__ __forceinline__ void flush_rows(int n, int parts, int p, int &lo, int &hi) {
+    const int ntile = (n + 15) / 16;
+    const int per = (ntile + parts - 1) / parts;
+    lo = min(n, 16 * per * p);
+    hi = min(n, 16 * per * (p + 1));
+}
+
 template <int NT, int CPL>
 __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev P, const int32_t *__restrict__ rep_list,
-                                                        int lds_bytes) {
+                                                        int lds_bytes, FlushHelp fh) {
     constexpr int NW = NT / kWave;
     extern __shared__ __align__(16) int32_t smem[];
 
-    const int g = rep_list[blockIdx.x];
+    // blocks from fh.nrep on are flush helpers: helper h of replica block r is block
+    // fh.nrep (1 + h) + r (the same XCD as r when fh.nrep is a multiple of 8)
+    const bool helper = fh.helpers > 0 && (int)blockIdx.x >= fh.nrep;
+    const int rb = helper ? ((int)blockIdx.x - fh.nrep) % fh.nrep : (int)blockIdx.x;
+    const int g = rep_list[rb];
     const int inst = P.rep_inst[g];
-    if (!P.state[inst].active) return;
+    if (!P.state[inst].active) return;  // the replica and its helpers alike
     const InstDesc d = P.inst[inst];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int n = d.n, ld = d.ld, n4 = round8(n);
@@ -105,6 +123,50 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
     const int kmax = lay.kmax;
     const int kpad = lay.kstride;  // U / V row stride in int16
     if (kmax < 8) return;          // the host places an instance here only when 8 updates fit
+    uint32_t *req_w = fh.words + 1 + 3 * rb, *k_w = req_w + 1, *done_w = req_w + 2;
+    const int16_t *uvg = fh.uv + fh.uv_stride * rb;
+
+    if (helper) {
+        // ---- flush helper: wait for a request, update this helper's rows, arrive ----
+        int32_t *Ig = P.I + d.I_base + (int64_t)local * n * ld;
+        uint32_t *slot = reinterpret_cast<uint32_t *>(smem);
+        int lo = 0, hi = 0;
+        flush_rows(n, fh.helpers + 1, 1 + ((int)blockIdx.x - fh.nrep) / fh.nrep, lo, hi);
+        uint32_t seen = 0;
+        for (;;) {
+            if (tid == 0) {
+                const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+                uint32_t v;
+                // relaxed polls: an acquire on every poll would invalidate the caches the
+                // replicas beside this helper read from; one acquire fence follows the change
+                while ((v = __hip_atomic_load(req_w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ==
+                       seen) {
+                    __builtin_amdgcn_s_sleep(8);
+                    if (__builtin_amdgcn_s_memrealtime() - t0 > kFlushWaitTicks) {
+                        __hip_atomic_store(fh.words, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        v = kFlushStop;
+                        break;
+                    }
+                }
+                slot[0] = v;
+                slot[1] = __hip_atomic_load(k_w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            const uint32_t v = slot[0];
+            const int K = (int)slot[1];
+            __syncthreads();
+            if (v == kFlushStop) return;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (lo < hi)
+                mfma_rank_update<NW, 1, kLazyFlushDepth<NT>>(Ig, ld, n, uvg, uvg + (size_t)n4 * kpad,
+                                                              kpad, K, lane, wid, lo, hi);
+            // every wave releases its own rows and counts itself
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0)
+                __hip_atomic_fetch_add(done_w, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            seen = v;
+        }
+    }
 
     int32_t *p_s = smem;
     int32_t *pinv_s = p_s + n4;
@@ -169,10 +231,45 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
     bool stale = false;  // a flush made the prefetched I_base row / column out of date
 
     // I_base += U V^T over the whole matrix on the matrix cores, then no pending update is
-    // left
+    // left.  With helpers the rows are split: U, V and K go to global memory, a request
+    // wakes the helpers, this workgroup updates the first run of rows, and every helper
+    // wave's arrival is awaited before the matrix is read again.
+    uint32_t fseq = 0;  // flushes requested from the helpers in this launch
     auto flush = [&]() {
         if (K == 0) return;
-        mfma_rank_update<NW, 1, kLazyFlushDepth<NT>>(Ig, ld, n, U_s, V_s, kpad, K, lane, wid);
+        if (fh.helpers > 0) {
+            int16_t *uvw = fh.uv + fh.uv_stride * rb;
+            const v4h *src = reinterpret_cast<const v4h *>(U_s);
+            v4h *dst = reinterpret_cast<v4h *>(uvw);
+            const int words = n4 * kpad / 8 * 2;  // U and V are contiguous, 8 int16 per word
+            for (int x = tid; x < words; x += NT) dst[x] = src[x];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            ++fseq;
+            if (tid == 0) {
+                __hip_atomic_store(k_w, (uint32_t)K, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(req_w, fseq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            int lo = 0, hi = 0;
+            flush_rows(n, fh.helpers + 1, 0, lo, hi);
+            mfma_rank_update<NW, 1, kLazyFlushDepth<NT>>(Ig, ld, n, U_s, V_s, kpad, K, lane, wid,
+                                                          lo, hi);
+            if (tid == 0) {
+                const uint32_t want = fseq * (uint32_t)(fh.helpers * (NT / kWave));
+                const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+                while (__hip_atomic_load(done_w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (__builtin_amdgcn_s_memrealtime() - t0 > kFlushWaitTicks) {
+                        __hip_atomic_store(fh.words, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        } else {
+            mfma_rank_update<NW, 1, kLazyFlushDepth<NT>>(Ig, ld, n, U_s, V_s, kpad, K, lane, wid);
+        }
         __syncthreads();  // the matrix writes reach every wave before it is read again
         zero_pending();
         K = 0;
@@ -355,6 +452,8 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
         }
     }
     flush();
+    if (fh.helpers > 0 && tid == 0)  // the helpers leave
+        __hip_atomic_store(req_w, kFlushStop, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 
     if (lane == 0) reinterpret_cast<uint64_t *>(red_s + 140)[wid] = cells;
     lds_sync<NW>();
@@ -378,32 +477,76 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
     }
 }
 
+// mode 0: plain launch of nblocks; 1: cooperative launch of fh.nrep (1 + fh.helpers)
+// blocks; 2: nothing launched, hipSuccess when that many blocks are resident at once
 template <int NT, int CPL>
 hipError_t launch_lazy_t(const PoolDev &P, const int32_t *list, int nblocks, int lds_bytes,
-                         hipStream_t s) {
+                         const FlushHelp &fh, int mode, hipStream_t s) {
     auto k = sweep_lazy_kernel<NT, CPL>;
     const hipError_t e = allow_full_lds<sweep_lazy_kernel<NT, CPL>>();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds_bytes, s, P, list, lds_bytes);
-    return hipGetLastError();
+    if (mode == 0) {
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(NT), lds_bytes, s, P, list, lds_bytes, fh);
+        return hipGetLastError();
+    }
+    const int blocks = fh.nrep * (1 + fh.helpers);
+    if (mode == 2) {
+        int per_cu = 0, dev = 0, cus = 0;
+        hipError_t q = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(k), NT, (size_t)lds_bytes);
+        if (q == hipSuccess) q = hipGetDevice(&dev);
+        if (q == hipSuccess) q = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (q != hipSuccess) return q;
+        return (int64_t)per_cu * cus >= blocks ? hipSuccess : hipErrorCooperativeLaunchTooLarge;
+    }
+    PoolDev pd = P;
+    const int32_t *lp = list;
+    int lb = lds_bytes;
+    FlushHelp f = fh;
+    void *args[] = {&pd, &lp, &lb, &f};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k), dim3(blocks), dim3(NT),
+                                      args, (unsigned)lds_bytes, s);
+}
+
+static hipError_t lazy_dispatch(const PoolDev &P, const SweepShape &sh, const int32_t *list,
+                                int nblocks, int lds_bytes, const FlushHelp &fh, int mode,
+                                hipStream_t s) {
+    if (!sh.sym) return hipErrorInvalidValue;
+    switch (sh.nt * 8 + sh.cpl) {
+    case 64 * 8 + 1: return launch_lazy_t<64, 1>(P, list, nblocks, lds_bytes, fh, mode, s);
+    case 64 * 8 + 2: return launch_lazy_t<64, 2>(P, list, nblocks, lds_bytes, fh, mode, s);
+    case 64 * 8 + 4: return launch_lazy_t<64, 4>(P, list, nblocks, lds_bytes, fh, mode, s);
+    case 128 * 8 + 1: return launch_lazy_t<128, 1>(P, list, nblocks, lds_bytes, fh, mode, s);
+    case 128 * 8 + 2: return launch_lazy_t<128, 2>(P, list, nblocks, lds_bytes, fh, mode, s);
+    case 256 * 8 + 1: return launch_lazy_t<256, 1>(P, list, nblocks, lds_bytes, fh, mode, s);
+    case 256 * 8 + 2: return launch_lazy_t<256, 2>(P, list, nblocks, lds_bytes, fh, mode, s);
+    case 256 * 8 + 4: return launch_lazy_t<256, 4>(P, list, nblocks, lds_bytes, fh, mode, s);
+    case 512 * 8 + 1: return launch_lazy_t<512, 1>(P, list, nblocks, lds_bytes, fh, mode, s);
+    case 512 * 8 + 2: return launch_lazy_t<512, 2>(P, list, nblocks, lds_bytes, fh, mode, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_sweep_lazy(const PoolDev &P, const SweepShape &sh, const int32_t *list,
                              int nblocks, int lds_bytes, hipStream_t s) {
-    if (!sh.sym) return hipErrorInvalidValue;
-    switch (sh.nt * 8 + sh.cpl) {
-    case 64 * 8 + 1: return launch_lazy_t<64, 1>(P, list, nblocks, lds_bytes, s);
-    case 64 * 8 + 2: return launch_lazy_t<64, 2>(P, list, nblocks, lds_bytes, s);
-    case 64 * 8 + 4: return launch_lazy_t<64, 4>(P, list, nblocks, lds_bytes, s);
-    case 128 * 8 + 1: return launch_lazy_t<128, 1>(P, list, nblocks, lds_bytes, s);
-    case 128 * 8 + 2: return launch_lazy_t<128, 2>(P, list, nblocks, lds_bytes, s);
-    case 256 * 8 + 1: return launch_lazy_t<256, 1>(P, list, nblocks, lds_bytes, s);
-    case 256 * 8 + 2: return launch_lazy_t<256, 2>(P, list, nblocks, lds_bytes, s);
-    case 256 * 8 + 4: return launch_lazy_t<256, 4>(P, list, nblocks, lds_bytes, s);
-    case 512 * 8 + 1: return launch_lazy_t<512, 1>(P, list, nblocks, lds_bytes, s);
-    case 512 * 8 + 2: return launch_lazy_t<512, 2>(P, list, nblocks, lds_bytes, s);
-    default: return hipErrorInvalidValue;
-    }
+    return lazy_dispatch(P, sh, list, nblocks, lds_bytes, FlushHelp{}, 0, s);
+}
+
+hipError_t launch_sweep_lazy_helped(const PoolDev &P, const SweepShape &sh, const int32_t *list,
+                                    int lds_bytes, const FlushHelp &fh, hipStream_t s) {
+    if (fh.helpers < 1 || fh.nrep < 1 || !fh.uv || !fh.words) return hipErrorInvalidValue;
+    return lazy_dispatch(P, sh, list, fh.nrep, lds_bytes, fh, 1, s);
+}
+
+bool lazy_helpers_fit(const SweepShape &sh, int nblocks, int helpers, int lds_bytes) {
+    FlushHelp fh;
+    fh.nrep = nblocks;
+    fh.helpers = helpers;
+    return lazy_dispatch(PoolDev{}, sh, nullptr, nblocks, lds_bytes, fh, 2, nullptr) == hipSuccess;
+}
+
+int64_t lazy_pending_elems(int n, int lds_bytes) {
+    return 2 * (int64_t)round8(n) * lazy_layout(n, lds_bytes).kstride;
 }
 
 }  // namespace qap

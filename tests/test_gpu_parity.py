@@ -174,6 +174,15 @@ CASES = {
     # force matrix in HBM with rank-2 updates batched in LDS (symmetric instances)
     "lazy_n40_R4": (lambda q: q.instances.random_instance(29, 40), 4, 12,
                     {"env": {"QAP_PT_IMODE": "3"}}),
+    # flush helpers: three helper workgroups per replica share every flush (rows split
+    # four ways, pending updates read from global memory)
+    "lazy_n40_R4_3helpers": (lambda q: q.instances.random_instance(29, 40), 4, 12,
+                             {"env": {"QAP_PT_IMODE": "3", "QAP_PT_FLUSH_HELPERS": "3"}}),
+    "lazy_n200_T2_2helpers": (lambda q: q.instances.mixed_instance(31, 200, True), 3, 4,
+                              {"T": 2, "kick_after": 1,
+                               "env": {"QAP_PT_IMODE": "3", "QAP_PT_FLUSH_HELPERS": "2"}}),
+    "lazy_c5_n512_1helper": (lambda q: q.instances.large_c5(512), 2, 1,
+                             {"env": {"QAP_PT_FLUSH_HELPERS": "1"}}),
     "lazy_sparse_n90_64x2": (lambda q: q.instances.mixed_instance(30, 90, False), 4, 8,
                              {"env": {"QAP_PT_IMODE": "3", "QAP_PT_SHAPE": "64,2"}}),
     "lazy_grey_n256_R3": (lambda q: q.instances.grey_c2(), 3, 3, {"env": {"QAP_PT_IMODE": "3"}}),
