@@ -105,6 +105,15 @@ int qap_pt_device_count(void);
 /* --- one-shot solve: replaces CPU_PIM/GPU_PIM main()'s solve path ------------- */
 int qap_pt_solve_hip(const qap_problem *prob, const qap_pt_params *par, qap_pt_result *res);
 
+/* --- the same solve on the host's cores: replaces CPU-PIM's own solve path
+ *     (CPU-PIM/CPU_PIM.cpp:478-685, setup -> threadToRun x T).  Same arguments, errors
+ *     and results as qap_pt_solve_hip (the same chain, update for update, bit-identical
+ *     for any thread count); makes no device call, so it runs on hosts without a GPU.
+ *     A ladder's replicas are spread over the threads (QAP_PT_CPU_THREADS, default every
+ *     core the process may run on, at most R*T).  QAP_PT_ONE_SHOT is a device-only
+ *     dynamics: QAP_PT_E_INVAL. ----------------------------------------------------- */
+int qap_pt_solve_cpu(const qap_problem *prob, const qap_pt_params *par, qap_pt_result *res);
+
 /* --- allocation-free solve.  No counterpart in the reference: it is for callers that
  *     cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc).
  *     The same solve as qap_pt_solve_hip -- same arguments, results (bit-identical) and

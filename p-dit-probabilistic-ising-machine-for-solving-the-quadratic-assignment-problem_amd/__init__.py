@@ -38,6 +38,7 @@ EXPORTS = (
     "qap_pt_last_error",
     "qap_pt_device_count",
     "qap_pt_solve_hip",
+    "qap_pt_solve_cpu",
     "qap_pt_solve_hip_noalloc",
     "qap_pt_solve_batch_hip",
     "qap_pt_pool_create",
@@ -137,6 +138,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.qap_pt_last_error.restype = C.c_char_p
     lib.qap_pt_device_count.restype = C.c_int
     lib.qap_pt_solve_hip.argtypes = [C.POINTER(QapProblem), C.POINTER(QapPtParams),
+                                     C.POINTER(QapPtResult)]
+    lib.qap_pt_solve_cpu.argtypes = [C.POINTER(QapProblem), C.POINTER(QapPtParams),
                                      C.POINTER(QapPtResult)]
     lib.qap_pt_solve_hip_noalloc.argtypes = [C.POINTER(QapProblem), C.POINTER(QapPtParams),
                                              C.POINTER(QapPtResult), C.c_void_p]
@@ -299,6 +302,29 @@ def solve(inst: Instance) -> dict:
     res = QapPtResult()
     res.best_perm = _ptr(perm, C.c_int32)
     _check(lib.qap_pt_solve_hip(C.byref(prob), C.byref(par), C.byref(res)))
+    return _result_dict(res, perm)
+
+
+def solve_cpu(inst: Instance, threads: int | None = None) -> dict:
+    """qap_pt_solve_cpu: the same solve on the host's cores (no device call); identical
+    results to `solve`.  threads sets QAP_PT_CPU_THREADS for the call (None = the
+    environment's value, else every core the process may run on)."""
+    lib = load_library()
+    prob, par = inst.c_structs()
+    perm = np.zeros(inst.n, dtype=np.int32)
+    res = QapPtResult()
+    res.best_perm = _ptr(perm, C.c_int32)
+    old = os.environ.get("QAP_PT_CPU_THREADS")
+    if threads is not None:
+        os.environ["QAP_PT_CPU_THREADS"] = str(int(threads))
+    try:
+        _check(lib.qap_pt_solve_cpu(C.byref(prob), C.byref(par), C.byref(res)))
+    finally:
+        if threads is not None:
+            if old is None:
+                os.environ.pop("QAP_PT_CPU_THREADS", None)
+            else:
+                os.environ["QAP_PT_CPU_THREADS"] = old
     return _result_dict(res, perm)
 
 

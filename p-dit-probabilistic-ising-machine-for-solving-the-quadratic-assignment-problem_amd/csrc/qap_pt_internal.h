@@ -15,6 +15,10 @@ namespace qap {
 // Sets qap_pt_last_error() for this thread and returns code.
 int fail(int code, const char *fmt, ...);
 
+// Checks one instance's arguments (index k in messages): QAP_PT_OK, or the error code with
+// qap_pt_last_error() set.  Shared by every solve entry point.
+int validate(const qap_problem &pb, const qap_pt_params &pr, int k);
+
 int pool_create_internal(int32_t count, const qap_problem *problems, const qap_pt_params *params,
                          qap_pt_pool **out);
 // Enqueues every instance's remaining sweeps (params.sweeps) on the pool stream as
