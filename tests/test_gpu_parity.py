@@ -260,6 +260,17 @@ CASES = {
     "wide64_n130_256x1_spe2": (lambda q: _wide64(84, 130, symmetric=False), 2, 4,
                                {"spe": 2, "beta_scale": _wide_scale(130),
                                 "expect_shape": "256x1,hbm64,general"}),
+    # persistent launches of several waves: 2, 4 and 8 candidates per lane in the pick,
+    # int32 and uint16 placements, symmetric and general updates
+    "lat_asym_n90_128x1": (lambda q: q.instances.random_instance(70, 90, 0, 30, symmetric=False),
+                           4, 8, {"env": {"QAP_PT_SHAPE": "128,1"}}),
+    "lat_sym_n200_128x2": (lambda q: q.instances.random_instance(71, 200), 3, 4,
+                           {"env": {"QAP_PT_SHAPE": "128,2"}}),
+    "lat_asym_diag_n240_256x2_lds16": (lambda q: q.instances.random_instance(
+        72, 240, 0, 12, symmetric=False, zero_diag=False), 3, 3,
+        {"env": {"QAP_PT_SHAPE": "256,2", "QAP_PT_IMODE": "1"}}),
+    "lat_sparse_n150_T2_kick": (lambda q: q.instances.mixed_instance(73, 150, False), 4, 12,
+                                {"T": 2, "kick_after": 2}),
     # the benchmark configurations at their own sizes, shapes and ladders
     # n = 1024 large dense: lazy placement, eight waves x 2 candidates, 24 pending updates
     "lazy_c5_n1024_R2": (lambda q: q.instances.large_c5(1024), 2, 1,
