@@ -113,8 +113,21 @@ hipError_t launch_oneshot(const PoolDev &P, const SweepShape &sh, const int32_t 
 hipError_t launch_exchange(const PoolDev &P, int count, int max_rt, hipStream_t s);
 hipError_t launch_set_limits(InstState *state, const int64_t *limits, int count, hipStream_t s);
 hipError_t launch_halt(InstState *state, int count, hipStream_t s);
+// Set-up force matrices of nrep replicas on the int8 matrix cores: replicas g .. g + group - 1
+// (g a multiple of group) start from the permutation of replica g, built once and stored to
+// each.  planes_d / planes_f: signed base-256 digits of max|D| / max|F|
+// (force_digit_planes, 1..4); the entries must fit int32 (force bound).  ld % 4 == 0,
+// ld > n.
+int force_digit_planes(int64_t max_abs);
+hipError_t launch_force_mfma(const int32_t *F, const int32_t *D, const int32_t *Ft,
+                             const int32_t *Dt, const int32_t *perm, int n, int ld, int nrep,
+                             int group, int planes_d, int planes_f, int32_t *Iout, hipStream_t s);
 hipError_t launch_force(const int32_t *F, const int32_t *D, const int32_t *perm, int n, int ld,
                         int nrep, int32_t *Iout, hipStream_t s);
+// Set-up costs of nrep replicas from their force matrices (E = -sum_i I[i][p(i)] / 2),
+// once per group of replicas sharing a permutation.
+hipError_t launch_energy_from_force(const int32_t *I, const int32_t *perm, int n, int ld, int nrep,
+                                   int group, int64_t *Eout, hipStream_t s);
 hipError_t launch_energy(const int32_t *F, const int32_t *D, const int32_t *perm, int n, int nrep,
                          int64_t *Eout, hipStream_t s);
 hipError_t launch_count_mismatch(const int32_t *a, const int32_t *b, int n, int ld,

@@ -617,6 +617,11 @@ void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throu
         h.beta2f.push_back((float)b2);
         h.c20.push_back(20.0 / b2);
     }
+    // replicas sharing a start: all of them with init_perm, each alone with per-replica
+    // starts, else the R replicas of a ladder
+    h.fb.planes_d = force_digit_planes(bnd.maxD);
+    h.fb.planes_f = force_digit_planes(bnd.maxF);
+    h.fb.group = pr.init_perm ? pr.R * pr.T : ((pr.flags & QAP_PT_INIT_PER_REPLICA) ? 1 : pr.R);
     std::vector<int32_t> pi(n), inv(n);
     for (int t = 0; t < pr.T; ++t) {
         if (!(pr.flags & QAP_PT_INIT_PER_REPLICA)) {
@@ -634,6 +639,26 @@ void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throu
             h.pos.push_back(l);
         }
     }
+}
+
+hipError_t enqueue_force_build(const qap_pt_pool *p, const InstDesc &d, const ForceBuild &fb,
+                               hipStream_t s) {
+    const int RT = d.R * d.T;
+    const char *env = std::getenv("QAP_PT_FORCE_BUILD");
+    if (env && std::strcmp(env, "valu") == 0) {
+        const hipError_t e = launch_force(p->d_F + d.mat_off, p->d_D + d.mat_off,
+                                          p->d_perm + d.p_base, d.n, d.ld, RT, p->d_I + d.I_base, s);
+        if (e != hipSuccess) return e;
+        return launch_energy(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n, RT,
+                             p->d_E + d.rep_base, s);
+    }
+    const hipError_t e = launch_force_mfma(p->d_F + d.mat_off, p->d_D + d.mat_off,
+                                           p->d_Ft + d.mat_off, p->d_Dt + d.mat_off,
+                                           p->d_perm + d.p_base, d.n, d.ld, RT, fb.group,
+                                           fb.planes_d, fb.planes_f, p->d_I + d.I_base, s);
+    if (e != hipSuccess) return e;
+    return launch_energy_from_force(p->d_I + d.I_base, p->d_perm + d.p_base, d.n, d.ld, RT,
+                                    fb.group, p->d_E + d.rep_base, s);
 }
 
 void order_bucket(const qap_pt_pool *p, Bucket &B, const std::function<int(int32_t)> &inst_of) {
@@ -724,6 +749,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     p->params.assign(params, params + count);
     p->desc.resize(count);
     p->shapes.resize(count);
+    p->fbuild.resize(count);
     p->limits.assign(count, 0);
 
     std::vector<int32_t> hF, hD, hFt, hDt, hFd, hDd, hCls, hSpan, rep_inst, perm, pinv, pos, order;
@@ -740,6 +766,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
         InstDesc &d = p->desc[k];
         d = h.d;
         p->shapes[k] = h.shape;
+        p->fbuild[k] = h.fb;
         const int n = d.n, RT = d.R * d.T;
         d.mat_off = p->total_mat;
         d.vec_off = p->total_vec;
@@ -935,14 +962,14 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     for (int k = 0; k < count; ++k) {
         const InstDesc &d = p->desc[k];
         const int RT = d.R * d.T;
-        if (p->shapes[k].im == IM_HBM64)
+        if (p->shapes[k].im == IM_HBM64) {
             POOL_TRY(launch_force64(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base,
                                     d.n, d.ld, RT, p->d_I64 + d.I64_base, p->stream));
-        else
-            POOL_TRY(launch_force(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base,
-                                  d.n, d.ld, RT, p->d_I + d.I_base, p->stream));
-        POOL_TRY(launch_energy(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, d.n,
-                               RT, p->d_E + d.rep_base, p->stream));
+            POOL_TRY(launch_energy(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base,
+                                   d.n, RT, p->d_E + d.rep_base, p->stream));
+        } else {
+            POOL_TRY(enqueue_force_build(p, d, p->fbuild[k], p->stream));
+        }
     }
     POOL_TRY(hipStreamSynchronize(p->stream));
 #undef POOL_TRY

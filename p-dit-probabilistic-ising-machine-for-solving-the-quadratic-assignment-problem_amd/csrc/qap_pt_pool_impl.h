@@ -14,13 +14,26 @@
 namespace qap {
 
 // Everything one instance needs on the device, built on the host (prepare_instance).
+// Set-up force build of one instance: signed base-256 digits of max|D| and max|F| for the
+// matrix cores, and how many consecutive replicas share each starting permutation.
+struct ForceBuild {
+    int planes_d = 1, planes_f = 1, group = 1;
+};
+
 struct HostInstance {
     InstDesc d{};
     SweepShape shape{};
+    ForceBuild fb{};
     std::vector<int32_t> F, D, Ft, Dt, Fd, Dd, cls, span, perm, pinv, pos;
     std::vector<float> beta2f;
     std::vector<double> c20;
 };
+
+// Enqueues the set-up force matrices and costs of an instance's RT replicas (int32
+// placements): the int8 matrix-core build and costs read off the force matrices, or with
+// QAP_PT_FORCE_BUILD=valu the vector-ALU build and the O(n^2) cost sums.
+hipError_t enqueue_force_build(const qap_pt_pool *p, const InstDesc &d, const ForceBuild &fb,
+                               hipStream_t s);
 
 struct Bucket {
     SweepShape shape;
@@ -39,6 +52,7 @@ struct qap_pt_pool {
     std::vector<qap_pt_params> params;
     std::vector<qap::InstDesc> desc;
     std::vector<qap::SweepShape> shapes;
+    std::vector<qap::ForceBuild> fbuild;  // per instance (slot)
     std::vector<qap::Bucket> buckets;
     std::vector<int64_t> limits;  // mega-iteration limit per instance (host mirror)
     int total_reps = 0, total_ladders = 0;

@@ -229,10 +229,7 @@ int slots_load(qap_pt_pool *p, int slot, const qap_problem &pb, const qap_pt_par
     QAP_HIP_TRY(up(p->d_inst + slot, &p->desc[slot], sizeof(InstDesc)));
     // force matrices (CPU_PIM.cpp:164-181) and costs (:184-192), then the state that
     // makes the slot live at the next exchange step
-    QAP_HIP_TRY(launch_force(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, n, d.ld,
-                             RT, p->d_I + d.I_base, s));
-    QAP_HIP_TRY(launch_energy(p->d_F + d.mat_off, p->d_D + d.mat_off, p->d_perm + d.p_base, n, RT,
-                              p->d_E + d.rep_base, s));
+    QAP_HIP_TRY(enqueue_force_build(p, d, h.fb, s));
     QAP_HIP_TRY(up(p->d_state + slot, &st, sizeof(InstState)));
     p->used[slot] = 1;
     p->lists_dirty = true;
