@@ -9,10 +9,10 @@
 //   row i      I[i][b] for lane b:  rowI[b] + dot(U[i], V[b])         (rowI = I_base row i)
 //   column a   I[j][a] for lane b:  colA[j] + dot(U[j], V[a])         (colA[x] = I_base[x][a])
 //   diagonal   dg[c] = I[pinv(c)][c], kept exact move by move; I[i][a] = dg[a].
-// When K reaches the capacity the LDS holds (kmax <= 32) the pending updates are applied to
-// I_base in one pass over the matrix, so kmax accepted moves cost one read and one write of
-// I instead of kmax; every launch ends with such a flush, so I in HBM is exact between
-// launches.  The flush is an integer matrix product U V^T on the matrix cores: each 16-bit
+// When K reaches the capacity the LDS holds (kmax <= 32, up to 64 from n = 384) the
+// pending updates are applied to I_base in one pass over the matrix, so kmax accepted moves
+// cost one read and one write of I instead of kmax; every launch ends with such a flush, so
+// I in HBM is exact between launches.  The flush is an integer matrix product U V^T on the matrix cores: each 16-bit
 // entry splits into two signed bytes, and the int8 products (exact in int32) are
 // recombined with shifts.  All sums are 32-bit two's complement: the entries themselves
 // fit (the host checks the force bound), so wrapped partial sums still give the exact
@@ -31,7 +31,14 @@ namespace qap {
 namespace {
 
 constexpr int kLazyRed = 160;  // reduction slots (same use as the other sweep kernel)
-constexpr int kLazyMaxK = 32;
+constexpr int kLazyMaxK = 64;
+// Below n = 384 at most 32 pending updates: the LDS of 32 leaves room for more workgroups
+// per CU, which the full-chip loads use.  From n = 384 on one workgroup holds its CU's LDS
+// anyway, and up to 64 pending updates (56 fit at n = 512) halve the flushes per sweep;
+// only the 512-thread shapes take more than 32 (the smaller ones keep their register
+// budget), so those instances run with 512 threads.
+constexpr int kLazyLargeN = 384;
+__host__ __device__ constexpr int lazy_cap(int n) { return n >= kLazyLargeN ? kLazyMaxK : 32; }
 // p, pinv, cls, rnd, dg, drow x2, frow x2, rowI x2, colA x2 -- round8(n) words each
 constexpr int kLazyArrays = 13;
 
@@ -48,7 +55,8 @@ struct LazyLayout {
 __host__ __device__ inline LazyLayout lazy_layout(int n, int lds_bytes) {
     const int avail = lds_bytes - 4 * lazy_fixed_words(n);
     const int g = avail > 0 ? avail / (4 * round8(n) * 8) : 0;  // 8-entry groups that fit
-    int kg = g < kLazyMaxK / 8 ? g : kLazyMaxK / 8;
+    const int cap = lazy_cap(n) / 8;
+    int kg = g < cap ? g : cap;
     int sg = kg | 1;
     if (sg > g) sg = --kg;  // kg was even and only kg groups fit: drop to kg - 1 (odd)
     return LazyLayout{8 * (kg > 0 ? kg : 0), 8 * (sg > 0 ? sg : 0)};
@@ -69,14 +77,14 @@ __device__ __forceinline__ int32_t dot8(v4h x, v4h y, int32_t acc) {
 }  // namespace
 
 int lazy_lds_bytes(int n, int max_pending) {
-    // max_pending entries (a multiple of 8, at most kLazyMaxK) in an odd number of 8-entry
+    // max_pending entries (a multiple of 8, at most lazy_cap(n)) in an odd number of 8-entry
     // groups
-    const int groups = (max_pending < kLazyMaxK ? max_pending : kLazyMaxK) / 8;
+    const int groups = (max_pending < lazy_cap(n) ? max_pending : lazy_cap(n)) / 8;
     const int want = 4 * lazy_fixed_words(n) + 4 * round8(n) * 8 * (groups | 1);
     return want < kMaxLdsBytes ? want : kMaxLdsBytes;
 }
 
-int lazy_lds_bytes(int n) { return lazy_lds_bytes(n, kLazyMaxK); }
+int lazy_lds_bytes(int n) { return lazy_lds_bytes(n, lazy_cap(n)); }
 
 bool lazy_fits(int n) { return lazy_layout(n, lazy_lds_bytes(n)).kmax >= 8; }
 
@@ -93,6 +101,22 @@ template <int NT> constexpr int kLazyFlushDepth = NT >= 512 ? kUpdDepth : 2;
 // reported error instead of a hang).
 constexpr uint64_t kFlushWaitTicks = 3000000000ull;
 constexpr uint32_t kFlushStop = 0xffffffffu;
+
+// Rows [lo, hi) of I_base += U V^T for K pending updates: one 32-entry chunk on the
+// matrix cores per pass, two when K > 32 (512-thread shapes only).
+template <int NT>
+__device__ __forceinline__ void lazy_flush_part(int32_t *Ig, int ld, int n, const int16_t *U,
+                                                const int16_t *V, int kpad, int K, int lane,
+                                                int wid, int lo, int hi) {
+    constexpr int NW = NT / kWave;
+    if constexpr (NT >= 512) {
+        if (K > 32) {
+            mfma_rank_update<NW, 2, kLazyFlushDepth<NT>>(Ig, ld, n, U, V, kpad, K, lane, wid, lo, hi);
+            return;
+        }
+    }
+    mfma_rank_update<NW, 1, kLazyFlushDepth<NT>>(Ig, ld, n, U, V, kpad, K, lane, wid, lo, hi);
+}
 
 // Rows [lo, hi) of part p of a flush split into `parts` runs of whole 16-row tiles.
 __device__ __forceinline__ void flush_rows(int n, int parts, int p, int &lo, int &hi) {
@@ -120,7 +144,8 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
     const int n = d.n, ld = d.ld, n4 = round8(n);
     const int local = g - d.rep_base;
     const LazyLayout lay = lazy_layout(n, lds_bytes);
-    const int kmax = lay.kmax;
+    // more than 32 pending updates: 512-thread shapes only (two 32-entry chunks per flush)
+    const int kmax = NT >= 512 ? lay.kmax : min(lay.kmax, 32);
     const int kpad = lay.kstride;  // U / V row stride in int16
     if (kmax < 8) return;          // the host places an instance here only when 8 updates fit
     uint32_t *req_w = fh.words + 1 + 3 * rb, *k_w = req_w + 1, *done_w = req_w + 2;
@@ -158,8 +183,8 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
             if (v == kFlushStop) return;
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             if (lo < hi)
-                mfma_rank_update<NW, 1, kLazyFlushDepth<NT>>(Ig, ld, n, uvg, uvg + (size_t)n4 * kpad,
-                                                              kpad, K, lane, wid, lo, hi);
+                lazy_flush_part<NT>(Ig, ld, n, uvg, uvg + (size_t)n4 * kpad, kpad, K, lane, wid, lo,
+                                    hi);
             // every wave releases its own rows and counts itself
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             if (lane == 0)
@@ -252,8 +277,7 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
             }
             int lo = 0, hi = 0;
             flush_rows(n, fh.helpers + 1, 0, lo, hi);
-            mfma_rank_update<NW, 1, kLazyFlushDepth<NT>>(Ig, ld, n, U_s, V_s, kpad, K, lane, wid,
-                                                          lo, hi);
+            lazy_flush_part<NT>(Ig, ld, n, U_s, V_s, kpad, K, lane, wid, lo, hi);
             if (tid == 0) {
                 const uint32_t want = fseq * (uint32_t)(fh.helpers * (NT / kWave));
                 const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
@@ -268,7 +292,7 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
             __syncthreads();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         } else {
-            mfma_rank_update<NW, 1, kLazyFlushDepth<NT>>(Ig, ld, n, U_s, V_s, kpad, K, lane, wid);
+            lazy_flush_part<NT>(Ig, ld, n, U_s, V_s, kpad, K, lane, wid, 0, n);
         }
         __syncthreads();  // the matrix writes reach every wave before it is read again
         zero_pending();
