@@ -16,6 +16,9 @@
 //     -al / -ah    automatic-mode proportions (:508-524)
 //     --seed S     Philox seed (the reference seeds from std::random_device)
 //     --tmin T --tmax T   geometric temperature ladder instead of -b/-m
+//     --cpu        run on the host's cores (qap_pt_solve_cpu: the same chain, no device
+//                  call), as CPU_PIM does; needs an explicit ladder (-b or --tmin/--tmax)
+//                  and writes the best energy once, at the end (no sweep<t>.txt)
 //
 // FILE holds n, then F (n*n), then D (n*n), whitespace separated (readProblem,
 // CPU_PIM.cpp:79-107).  Progress lines are written after every chunk of exchange
@@ -48,6 +51,7 @@ struct Options {
     unsigned long long seed = 0x5eedULL;
     double tmin = 0.0, tmax = 0.0;
     bool one_shot = false;
+    bool cpu = false;
 };
 
 bool read_problem(const std::string &path, int &n, std::vector<int32_t> &F, std::vector<int32_t> &D) {
@@ -110,6 +114,9 @@ bool auto_select(const Options &o, int n, const std::vector<int32_t> &F, const s
     return true;
 }
 
+int finish(const Options &o, int n, const qap_pt_result &res, const std::vector<int32_t> &best_perm,
+           std::ostream &log, std::chrono::steady_clock::time_point t0);
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -128,7 +135,8 @@ int main(int argc, char **argv) {
                       << "  -b -m -r -t -w -i -g -G -s -o -al -ah as the reference CPU_PIM;\n"
                       << "  --seed S, --tmin T --tmax T (geometric temperature ladder)\n"
                       << "  --one-shot: one-shot parallel-proposal steps (the GPU-CUPIM driver's\n"
-                      << "              dynamics) instead of p-dit sweeps; -i counts steps\n";
+                      << "              dynamics) instead of p-dit sweeps; -i counts steps\n"
+                      << "  --cpu: the same run on the host's cores (needs -b or --tmin/--tmax)\n";
             return 0;
         } else if (a == "-b") {
             o.beta0_ln = std::atof(next("-b"));
@@ -150,6 +158,7 @@ int main(int argc, char **argv) {
         else if (a == "--tmin") o.tmin = std::atof(next("--tmin"));
         else if (a == "--tmax") o.tmax = std::atof(next("--tmax"));
         else if (a == "--one-shot") o.one_shot = true;
+        else if (a == "--cpu") o.cpu = true;
         else o.file = a;
     }
     if (o.file.empty()) {
@@ -158,6 +167,12 @@ int main(int argc, char **argv) {
     }
     if (o.R < 1 || o.T < 1 || o.megas < 0 || o.numitera < 1) {
         std::cerr << "Error: -r, -t, -i must be >= 1 and -w >= 0." << std::endl;
+        return 1;
+    }
+    if (o.cpu && (o.one_shot || (!o.have_beta && !(o.tmin > 0 && o.tmax > 0)))) {
+        std::cerr << "Error: --cpu runs p-dit sweeps with an explicit ladder (-b, or --tmin and "
+                     "--tmax); the automatic scan and --one-shot run on the device."
+                  << std::endl;
         return 1;
     }
     if (o.outdir.empty()) o.outdir = o.file + ".out";
@@ -205,11 +220,18 @@ int main(int argc, char **argv) {
     par.goal = o.goal;
     par.timeout_s = o.timeout;
     par.flags = o.one_shot ? QAP_PT_ONE_SHOT : 0u;
-    qap_pt_pool *pool = nullptr;
-    if (check(qap_pt_pool_create(1, &prob, &par, &pool))) return 1;
     std::vector<int32_t> best_perm(n);
     qap_pt_result res{};
     res.best_perm = best_perm.data();
+    if (o.cpu) {  // one call on the host's cores; the best is reported at the end
+        if (check(qap_pt_solve_cpu(&prob, &par, &res))) return 1;
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        log << "Best Energy Found: " << res.best_cost << " Time (s): " << el << std::endl;
+        std::cout << "Best Energy Found: " << res.best_cost << " Time (s): " << el << std::endl;
+        return finish(o, n, res, best_perm, log, t0);
+    }
+    qap_pt_pool *pool = nullptr;
+    if (check(qap_pt_pool_create(1, &prob, &par, &pool))) return 1;
     int64_t shown = INT64_MAX;
     // sweep<t>.txt for every ladder t (CPU_PIM.cpp:320,660-664): "megas elapsed_ms best"
     // whenever the best over the ladder's replicas improved (replicas are ladder-major)
@@ -243,6 +265,15 @@ int main(int argc, char **argv) {
         if (o.timeout > 0 && el >= o.timeout) break;
     }
     qap_pt_pool_destroy(pool);
+    return finish(o, n, res, best_perm, log, t0);
+}
+
+namespace {
+
+// The closing lines (CPU_PIM.cpp:667-684): the permutation with -G, the single best energy,
+// the gap to the goal and the elapsed time.
+int finish(const Options &o, int n, const qap_pt_result &res, const std::vector<int32_t> &best_perm,
+           std::ostream &log, std::chrono::steady_clock::time_point t0) {
     const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (o.goal_print > 0 && res.best_cost < o.goal_print) {
         for (int i = 0; i < n; ++i) {
@@ -261,3 +292,5 @@ int main(int argc, char **argv) {
     std::cout << "Elapsed time: " << el << " seconds." << std::endl;
     return 0;
 }
+
+}  // namespace
