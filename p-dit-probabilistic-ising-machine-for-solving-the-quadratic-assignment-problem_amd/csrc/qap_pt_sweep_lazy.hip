@@ -62,15 +62,25 @@ __host__ __device__ inline LazyLayout lazy_layout(int n, int lds_bytes) {
     return LazyLayout{8 * (kg > 0 ? kg : 0), 8 * (sg > 0 ? sg : 0)};
 }
 
-// sum_k x[k] y[k] over packed signed 16-bit entries, 8 per 128-bit word: each half is
-// sign-extended and multiplied in 32-bit integer arithmetic (exact)
+// sum_k x[k] y[k] over packed signed 16-bit entries, 8 per 128-bit word, on the packed
+// dot-product instruction (v_dot2c_i32_i16: lo*lo + hi*hi + acc, wrapping in 32 bits
+// without the clamp: exactly the 32-bit sum the update rule needs).  The pairs are taken
+// as shuffles of the word viewed as 8 shorts: bit-casting the word's int elements one by
+// one to short2 in an unrolled loop was compiled to four copies of the first pair (a
+// device check of this helper against integer arithmetic found it; the shuffle, memcpy and
+// short2-pointer forms are exact).
+typedef short lazy_s2 __attribute__((ext_vector_type(2)));
+typedef short lazy_s8 __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ int32_t dot8(v4h x, v4h y, int32_t acc) {
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-        const int32_t xl = (int32_t)(int16_t)(x[h] & 0xffff), xh = x[h] >> 16;
-        const int32_t yl = (int32_t)(int16_t)(y[h] & 0xffff), yh = y[h] >> 16;
-        acc += xl * yl + xh * yh;
-    }
+    const lazy_s8 a = __builtin_bit_cast(lazy_s8, x), b = __builtin_bit_cast(lazy_s8, y);
+    acc = __builtin_amdgcn_sdot2(__builtin_shufflevector(a, a, 0, 1),
+                                 __builtin_shufflevector(b, b, 0, 1), acc, false);
+    acc = __builtin_amdgcn_sdot2(__builtin_shufflevector(a, a, 2, 3),
+                                 __builtin_shufflevector(b, b, 2, 3), acc, false);
+    acc = __builtin_amdgcn_sdot2(__builtin_shufflevector(a, a, 4, 5),
+                                 __builtin_shufflevector(b, b, 4, 5), acc, false);
+    acc = __builtin_amdgcn_sdot2(__builtin_shufflevector(a, a, 6, 7),
+                                 __builtin_shufflevector(b, b, 6, 7), acc, false);
     return acc;
 }
 
