@@ -128,8 +128,10 @@ def workload(pkg, name, sweeps=0):
 def algorithmic_bytes(att, acc, cells, n):
     """24 n bytes per attempt (six int32 n-vectors: force row, gathered force column,
     gathered force diagonal, distance row, flow row, inverse permutation) plus, per
-    accepted move, 16 n bytes for the two difference vectors and 8 bytes (int32 read +
-    write) per force entry the move rewrites."""
+    accepted move, 16 n bytes for the two difference vectors, plus 8 bytes (int32 read +
+    write) per force entry written: per entry a move rewrites where moves update the
+    matrix one by one, per entry a flush writes where accepted moves wait in LDS and up
+    to 64 of them are applied in one pass over the matrix (the lazy placement)."""
     return 24 * n * att + 16 * n * acc + 8 * cells
 
 
@@ -148,19 +150,22 @@ def roofline_pass(pool, insts, sweeps, traffic_name):
     best = max((b for b in buckets if b["kernel_ms"] >= 0.95 * top),
                key=lambda b: (b["replicas"], b["kernel_ms"]))
     key = ",".join(best["shape"].split(",")[:3])  # NTxCPL,placement,sym|general
+    placement = key.split(",")[1]
+    written = "flush_cells" if placement == "lazy" else "update_cells"
     alg = 0
+    by_move = 0  # the same moves' bytes had each updated the matrix itself
     members = 0
     for k, x in enumerate(insts):
         if ",".join(shapes[k].split(",")[:3]) != key:
             continue
         members += 1
-        alg += algorithmic_bytes(after[k]["attempts"] - before[k]["attempts"],
-                                 after[k]["accepted"] - before[k]["accepted"],
-                                 after[k]["update_cells"] - before[k]["update_cells"], x.n)
+        d = {f: after[k][f] - before[k][f]
+             for f in ("attempts", "accepted", "update_cells", "flush_cells")}
+        alg += algorithmic_bytes(d["attempts"], d["accepted"], d[written], x.n)
+        by_move += algorithmic_bytes(d["attempts"], d["accepted"], d["update_cells"], x.n)
     launches = max(best["launches"], 1)
     avg_s = best["kernel_ms"] / 1e3 / launches
     achieved = alg / launches / avg_s / 1e9
-    placement = key.split(",")[1]
     lds = placement in ("lds32", "lds16")
     peak = LDS_PEAK_GBS if lds else HBM_PEAK_GBS
     # HBM bytes per launch from the PMC record of this workload and kernel, kept per
@@ -187,8 +192,11 @@ def roofline_pass(pool, insts, sweeps, traffic_name):
         "intervals_per_launch": intervals_per_launch,
         "instances_in_kernel": members,
         "algorithmic_bytes_per_launch": alg / launches,
+        "written_entries": written,
         "peak_basis": ("LDS: 256 CUs x 128 B/clk x 2.4 GHz" if lds else "HBM3E 8 TB/s"),
     }
+    if placement == "lazy":
+        out["move_by_move_bytes_per_launch"] = by_move / launches
     if traffic is not None:  # counter HBM traffic against the HBM roof, beside the bound
         out["hbm_traffic_gbs"] = traffic / avg_s / 1e9
         out["hbm_traffic_frac"] = traffic / avg_s / 1e9 / HBM_PEAK_GBS

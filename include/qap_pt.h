@@ -92,6 +92,10 @@ typedef struct {
     uint64_t kicks;              /* hot/cold kicks                                         */
     uint64_t update_cells;       /* force-matrix entries rewritten by rank-2 updates
                                     (columns with a zero flow difference are skipped)      */
+    uint64_t flush_cells;        /* force-matrix entries written by flushes that apply up to
+                                    64 pending rank-2 updates in one pass (placement of
+                                    symmetric instances whose matrix stays in HBM); 0 where
+                                    every move updates the matrix itself                   */
     int64_t sweeps_done;         /* per replica                                            */
     double seconds;              /* wall time of the sweep phase                          */
 } qap_pt_result;

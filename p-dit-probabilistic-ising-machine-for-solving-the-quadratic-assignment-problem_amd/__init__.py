@@ -101,6 +101,7 @@ class QapPtResult(C.Structure):
         ("exchanges", C.c_uint64),
         ("kicks", C.c_uint64),
         ("update_cells", C.c_uint64),
+        ("flush_cells", C.c_uint64),
         ("sweeps_done", C.c_int64),
         ("seconds", C.c_double),
     ]
@@ -289,6 +290,7 @@ def _result_dict(res: QapPtResult, perm: np.ndarray) -> dict:
         "exchanges": int(res.exchanges),
         "kicks": int(res.kicks),
         "update_cells": int(res.update_cells),
+        "flush_cells": int(res.flush_cells),
         "sweeps_done": int(res.sweeps_done),
         "seconds": float(res.seconds),
     }

@@ -470,6 +470,7 @@ struct Solver {
         res->attempts = (uint64_t)res->sweeps_done * (uint64_t)n * (uint64_t)RT;
         res->accepted = 0;
         res->update_cells = 0;
+        res->flush_cells = 0;  // updates are applied move by move here
         for (const Replica &r : reps) {
             res->accepted += r.accepted;
             res->update_cells += r.cells;

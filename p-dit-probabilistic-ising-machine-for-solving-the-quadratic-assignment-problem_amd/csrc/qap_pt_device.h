@@ -146,6 +146,8 @@ struct PoolDev {
     int64_t *E, *bestE;
     uint64_t *accepted;
     uint64_t *cells;          // force-matrix entries rewritten by rank-2 updates
+    uint64_t *fcells;         // force-matrix entries written by flushes of pending updates
+                              //   (lazy placement; the second half of the cells buffer)
     int32_t *pos;             // ladder position of each replica
     int32_t *order;           // per ladder: position -> replica (local index)
     int32_t *tss;             // per ladder: timeSinceSwap

@@ -706,6 +706,7 @@ void bind_pool(qap_pt_pool *p) {
     P.bestE = p->d_bestE;
     P.accepted = p->d_accepted;
     P.cells = p->d_cells;
+    P.fcells = p->d_cells + p->total_reps;
     P.pos = p->d_pos;
     P.order = p->d_order;
     P.tss = p->d_tss;
@@ -894,7 +895,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(palloc(p, &p->d_E, RTall));
     POOL_TRY(palloc(p, &p->d_bestE, RTall));
     POOL_TRY(palloc(p, &p->d_accepted, RTall));
-    POOL_TRY(palloc(p, &p->d_cells, RTall));
+    POOL_TRY(palloc(p, &p->d_cells, 2 * (size_t)RTall));  // update and flush counts
     POOL_TRY(palloc(p, &p->d_pos, RTall));
     POOL_TRY(palloc(p, &p->d_order, RTall));
     POOL_TRY(palloc(p, &p->d_tss, p->total_ladders));
@@ -946,6 +947,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(up(p->d_bestE, bestE.data(), 8 * bestE.size()));
     POOL_TRY(up(p->d_accepted, acc.data(), 8 * acc.size()));
     POOL_TRY(up(p->d_cells, acc.data(), 8 * acc.size()));
+    POOL_TRY(up(p->d_cells + RTall, acc.data(), 8 * acc.size()));
     POOL_TRY(up(p->d_pos, pos.data(), 4 * pos.size()));
     POOL_TRY(up(p->d_order, order.data(), 4 * order.size()));
     POOL_TRY(up(p->d_tss, tss.data(), 4 * tss.size()));
@@ -1119,12 +1121,14 @@ int pool_result_impl(qap_pt_pool *p, int inst, qap_pt_result *res) {
     const InstDesc &d = p->desc[inst];
     const int RT = d.R * d.T, n = d.n;
     std::vector<int64_t> E(RT), bestE(RT);
-    std::vector<uint64_t> acc(RT), cells(RT);
+    std::vector<uint64_t> acc(RT), cells(RT), fcells(RT);
     InstState st;
     HIP_TRY(hipMemcpy(E.data(), p->d_E + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(bestE.data(), p->d_bestE + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(acc.data(), p->d_accepted + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cells.data(), p->d_cells + d.rep_base, 8 * RT, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(fcells.data(), p->d_cells + p->total_reps + d.rep_base, 8 * RT,
+                      hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(&st, p->d_state + inst, sizeof(st), hipMemcpyDeviceToHost));
     int bi = -1;
     int64_t best = INT64_MAX;
@@ -1150,6 +1154,8 @@ int pool_result_impl(qap_pt_pool *p, int inst, qap_pt_result *res) {
     for (auto a : acc) res->accepted += a;
     res->update_cells = 0;
     for (auto c : cells) res->update_cells += c;
+    res->flush_cells = 0;
+    for (auto c : fcells) res->flush_cells += c;
     res->exchanges = st.exchanges;
     res->kicks = st.kicks;
     res->seconds = p->seconds;
@@ -1223,7 +1229,7 @@ int64_t pool_device_bytes(int32_t count, const qap_problem *problems, const qap_
         bytes += 4 * (4 * n * n + 4 * n);            // F, D, Ft, Dt, diagonals, classes, spans
         bytes += (wide_instance(problems[k]) ? 8 : 4) * RT * n * ld;  // force matrices
         bytes += 4 * 3 * RT * n;                      // perm, pinv, best_perm
-        bytes += RT * (8 * 4 + 4 * 3);                // E, bestE, accepted, cells, pos, order, rep_inst
+        bytes += RT * (8 * 5 + 4 * 3);                // E, bestE, accepted, 2 x cells, pos, order, rep_inst
         bytes += 24 * params[k].T + 16 * params[k].R + sizeof(InstDesc) + sizeof(InstState) + 8;
     }
     return bytes;

@@ -120,7 +120,7 @@ int slots_create(int32_t slots, int32_t n_cap, int32_t r_cap, int32_t t_cap, int
     SLOT_TRY(dalloc(&p->d_E, R));
     SLOT_TRY(dalloc(&p->d_bestE, R));
     SLOT_TRY(dalloc(&p->d_accepted, R));
-    SLOT_TRY(dalloc(&p->d_cells, R));
+    SLOT_TRY(dalloc(&p->d_cells, 2 * R));  // update and flush counts
     SLOT_TRY(dalloc(&p->d_pos, R));
     SLOT_TRY(dalloc(&p->d_order, R));
     SLOT_TRY(dalloc(&p->d_tss, p->total_ladders));
@@ -222,6 +222,7 @@ int slots_load(qap_pt_pool *p, int slot, const qap_problem &pb, const qap_pt_par
     QAP_HIP_TRY(up(p->d_bestE + d.rep_base, bestE, 8 * RT));
     QAP_HIP_TRY(up(p->d_accepted + d.rep_base, zeros, 8 * RT));
     QAP_HIP_TRY(up(p->d_cells + d.rep_base, zeros, 8 * RT));
+    QAP_HIP_TRY(up(p->d_cells + p->total_reps + d.rep_base, zeros, 8 * RT));
     QAP_HIP_TRY(up(p->d_pos + d.rep_base, h.pos.data(), 4 * RT));
     QAP_HIP_TRY(up(p->d_order + d.rep_base, h.pos.data(), 4 * RT));
     QAP_HIP_TRY(up(p->d_tss + d.ladder_base, zeros, 4 * d.T));

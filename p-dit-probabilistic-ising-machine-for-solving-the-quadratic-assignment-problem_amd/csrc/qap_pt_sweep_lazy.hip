@@ -251,6 +251,7 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
     int64_t bestE = P.bestE[g];
     uint64_t acc = P.accepted[g];
     uint64_t cells = 0;
+    uint64_t fcells = 0;  // entries written by flushes (the same count in every thread)
     const int pos = P.pos[g];
     const float beta2f = P.beta2f[d.beta_off + pos];
     const double c20 = P.c20[d.beta_off + pos];
@@ -272,6 +273,7 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
     uint32_t fseq = 0;  // flushes requested from the helpers in this launch
     auto flush = [&]() {
         if (K == 0) return;
+        fcells += (uint64_t)n * (uint64_t)n;
         if (fh.helpers > 0) {
             int16_t *uvw = fh.uv + fh.uv_stride * rb;
             const v4h *src = reinterpret_cast<const v4h *>(U_s);
@@ -508,6 +510,7 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
         P.bestE[g] = bestE;
         P.accepted[g] = acc;
         P.cells[g] += tot;
+        P.fcells[g] += fcells;
     }
 }
 

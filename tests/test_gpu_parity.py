@@ -306,6 +306,25 @@ def test_trajectory_bit_exact(qpt, oracle, gpu, name):
     _assert_same(res, st, ver, ref)
 
 
+def test_flush_cells_count(qpt, oracle, gpu):
+    """flush_cells counts the entries the lazy placement's flushes write (n^2 per flush):
+    between one flush per 64 accepted moves and one per accepted move, and at most one per
+    replica per launch beyond one per 8 moves; in-LDS placements update move by move and
+    report none.  Trajectories still equal the oracle's."""
+    F, D = qpt.instances.random_instance(29, 40)
+    n, R, sweeps = 40, 4, 12
+    betas = qpt.instances.geometric_ladder(R, 1.0, 100.0) / (n / 16.0)
+    res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, sweeps, env={"QAP_PT_IMODE": "3"})
+    _assert_same(res, st, ver, ref)
+    acc, fc = res["accepted"], res["flush_cells"]
+    assert acc > 0 and fc % (n * n) == 0, (acc, fc)
+    flushes = fc // (n * n)
+    assert acc / 64 <= flushes <= min(acc, R * sweeps + acc / 8), (acc, flushes)
+    res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, sweeps, env={"QAP_PT_IMODE": "0"})
+    _assert_same(res, st, ver, ref)
+    assert res["flush_cells"] == 0 and res["update_cells"] > 0
+
+
 def test_goal_stops_like_oracle(qpt, oracle, gpu):
     F, D = qpt.instances.random_instance(21, 30)
     betas = qpt.instances.geometric_ladder(6, 1.0, 50.0)
