@@ -66,6 +66,12 @@ __device__ __forceinline__ void lds_sync() {
 // cutoff, prefix sums in candidate order, and the owner of the first candidate whose
 // inclusive prefix exceeds floor(ri * S / 2^32).  red_s slots [0,16) and [16,80) carry
 // the per-wave partials; every lane returns S and whether it owns the winner.
+//
+// With `shift_s` set, the deltas arrive without one term that every candidate shares,
+// which is read from LDS (*shift_s) only after the first barrier: the pick subtracts it
+// from the deltas of the real candidates (those above INT32_MIN) and from their maximum.
+// A value written to *shift_s before the pick is then published by the pick's own
+// barrier, so the caller needs no barrier of its own for it.
 struct Pick {
     bool owner;
     int b, j;     // winning facility and its location (a and i for the stay)
@@ -74,10 +80,11 @@ struct Pick {
 };
 
 template <int NT, int CPL>
-__device__ __forceinline__ Pick heat_bath_pick(const int32_t (&dE)[CPL], int lmax,
+__device__ __forceinline__ Pick heat_bath_pick(int32_t (&dE)[CPL], int lmax,
                                                const int32_t (&pv)[CPL], uint32_t ri,
                                                float beta2f, double c20, int a, int i,
-                                               int32_t *red_s, int tid) {
+                                               int32_t *red_s, int tid,
+                                               const int32_t *shift_s = nullptr) {
     constexpr int NW = NT / kWave;
     const int lane = tid & (kWave - 1), wid = tid / kWave;
     int maxd;
@@ -92,6 +99,13 @@ __device__ __forceinline__ Pick heat_bath_pick(const int32_t (&dE)[CPL], int lma
 #pragma unroll
             for (int w = 1; w < NW; ++w) maxd = max(maxd, red_s[w]);
         }
+    }
+    if (shift_s) {  // lmax was taken over the unshifted deltas (and INT32_MIN lanes)
+        const int32_t sh = *shift_s;
+        maxd = max(0, maxd - sh);
+#pragma unroll
+        for (int q = 0; q < CPL; ++q)
+            if (dE[q] != INT32_MIN) dE[q] -= sh;
     }
     const int32_t cut = heat_cutoff(maxd, c20);
     uint32_t w[CPL], incl[CPL], qbase[CPL], wbase[CPL];

@@ -339,14 +339,15 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
             const int32_t *frow = frow_s + buf * n4;  // F[a][.]
             const int32_t *rowI = rowI_s + buf * n4;  // I_base[i][.]
             const int32_t *colA = colA_s + buf * n4;  // I_base[.][a]
-            const int32_t Iia = dg_s[a];
             const int groups = K / 8 + ((K & 7) != 0);
             const v4h *Ui = reinterpret_cast<const v4h *>(U_s + (size_t)i * kpad);
             const v4h *Va = reinterpret_cast<const v4h *>(V_s + (size_t)a * kpad);
 
-            // (1) deltas of every candidate from I_base plus the pending updates
+            // (1) deltas of every candidate from I_base plus the pending updates, without
+            // the term -I[i][a] = -dg[a] that all share: the pick subtracts it after its
+            // first barrier, which also publishes the diagonal entries the last move wrote
             int32_t dE[CPL], rv[CPL], cv[CPL];
-            int lmax = 0;
+            int lmax = INT32_MIN;
 #pragma unroll
             for (int q = 0; q < CPL; ++q) {
                 const int b = tid + q * NT;
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
                     }
                     rv[q] = r;  // I[i][b]
                     cv[q] = c;  // I[j][a]
-                    const int32_t v = r + c - Iia - dg_s[b] - 4 * drow[j] * frow[b];
+                    const int32_t v = r + c - dg_s[b] - 4 * drow[j] * frow[b];
                     dE[q] = v;
                     lmax = max(lmax, v);
                 }
@@ -384,7 +385,8 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
             }
 
             // (2)-(3) heat-bath pick; the winner commits the permutation swap
-            const Pick pk = heat_bath_pick<NT, CPL>(dE, lmax, pv, ri, beta2f, c20, a, i, red_s, tid);
+            const Pick pk =
+                heat_bath_pick<NT, CPL>(dE, lmax, pv, ri, beta2f, c20, a, i, red_s, tid, dg_s + a);
             int32_t my_rv = 0, my_cv = 0;
 #pragma unroll
             for (int q = 0; q < CPL; ++q)
@@ -467,7 +469,9 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
                         }
                     }
                     ++K;
-                    lds_sync<NW>();
+                    // no barrier: each lane wrote its own diagonal entries, and the one
+                    // another lane reads (dg[a] of the next site) waits for the pick's
+                    // first barrier
                     if (K == kmax) flush();
                 } else if (tid == 0) {
                     // identical flow rows and columns: I's columns a and b are equal, so the
