@@ -133,6 +133,8 @@ struct PoolDev {
     InstState *state;
     const int32_t *rep_inst;  // global replica -> instance
     const int32_t *F, *D, *Ft, *Dt;
+    const int16_t *F16, *D16; // F and D saturated to int16 (read by the lazy kernel, whose
+                              //   instances have every entry within 16 bits)
     const int32_t *Fdiag, *Ddiag;
     const int32_t *fclass;    // per facility: low 16 bits equal <=> identical flow row and
                               //   column; high 16 bits equal <=> twins (symmetric flow rows

@@ -259,6 +259,20 @@ hipError_t launch_halt(InstState *state, int count, hipStream_t s) {
     return hipGetLastError();
 }
 
+__global__ void narrow16_kernel(const int32_t *__restrict__ src, int16_t *__restrict__ dst,
+                                int64_t count) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count;
+         k += (int64_t)gridDim.x * blockDim.x)
+        dst[k] = (int16_t)max(-32768, min(32767, src[k]));
+}
+
+hipError_t launch_narrow16(const int32_t *src, int16_t *dst, int64_t count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, 4096);
+    hipLaunchKernelGGL(narrow16_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_force(const int32_t *F, const int32_t *D, const int32_t *perm, int n, int ld,
                         int nrep, int32_t *Iout, hipStream_t s) {
     const int tiles = (n + 15) / 16;

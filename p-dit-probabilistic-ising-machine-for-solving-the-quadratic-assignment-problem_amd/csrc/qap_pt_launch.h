@@ -130,6 +130,9 @@ hipError_t launch_energy_from_force(const int32_t *I, const int32_t *perm, int n
                                    int group, int64_t *Eout, hipStream_t s);
 hipError_t launch_energy(const int32_t *F, const int32_t *D, const int32_t *perm, int n, int nrep,
                          int64_t *Eout, hipStream_t s);
+// dst[k] = src[k] saturated to int16, k < count (the 16-bit copies of F and D that the lazy
+// kernel reads; every entry of an instance placed there fits).
+hipError_t launch_narrow16(const int32_t *src, int16_t *dst, int64_t count, hipStream_t s);
 hipError_t launch_count_mismatch(const int32_t *a, const int32_t *b, int n, int ld,
                                  int64_t total_rows, unsigned long long *out, hipStream_t s);
 

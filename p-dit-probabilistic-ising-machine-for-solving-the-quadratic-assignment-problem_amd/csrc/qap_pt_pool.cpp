@@ -348,7 +348,8 @@ void pool_free(qap_pt_pool *p) {
     }
     for (auto &b : p->buckets)
         if (b.d_list) (void)hipFree(b.d_list);
-    void *bufs[] = {p->d_inst,  p->d_state,     p->d_rep_inst, p->d_F,      p->d_D, p->d_fclass, p->d_fspan,
+    void *bufs[] = {p->d_inst,  p->d_state,     p->d_rep_inst, p->d_F,      p->d_D, p->d_F16, p->d_D16,
+                    p->d_fclass, p->d_fspan,
                     p->d_Ft,    p->d_Dt,        p->d_Fdiag,    p->d_Ddiag,  p->d_beta2f,
                     p->d_c20,   p->d_perm,      p->d_pinv,     p->d_best_perm, p->d_I, p->d_I64,
                     p->d_E,     p->d_bestE,     p->d_limits,   p->d_accepted, p->d_cells, p->d_pos,
@@ -689,6 +690,8 @@ void bind_pool(qap_pt_pool *p) {
     P.rep_inst = p->d_rep_inst;
     P.F = p->d_F;
     P.D = p->d_D;
+    P.F16 = p->d_F16;
+    P.D16 = p->d_D16;
     P.Ft = p->d_Ft;
     P.Dt = p->d_Dt;
     P.Fdiag = p->d_Fdiag;
@@ -879,6 +882,8 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(palloc(p, &p->d_rep_inst, RTall));
     POOL_TRY(palloc(p, &p->d_F, p->total_mat));
     POOL_TRY(palloc(p, &p->d_D, p->total_mat));
+    POOL_TRY(palloc(p, &p->d_F16, p->total_mat));
+    POOL_TRY(palloc(p, &p->d_D16, p->total_mat));
     POOL_TRY(palloc(p, &p->d_Ft, p->total_mat));
     POOL_TRY(palloc(p, &p->d_Dt, p->total_mat));
     POOL_TRY(palloc(p, &p->d_Fdiag, p->total_vec));
@@ -957,6 +962,8 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(hipMemsetAsync(p->d_I64, 0, sizeof(int64_t) * std::max<int64_t>(p->total_I64, 1),
                             p->stream));
     POOL_TRY(hipMemsetAsync(p->d_sync, 0, sizeof(uint32_t) * (1 + 2 * (size_t)count), p->stream));
+    POOL_TRY(launch_narrow16(p->d_F, p->d_F16, p->total_mat, p->stream));
+    POOL_TRY(launch_narrow16(p->d_D, p->d_D16, p->total_mat, p->stream));
 
     bind_pool(p);
 
@@ -1226,7 +1233,7 @@ int64_t pool_device_bytes(int32_t count, const qap_problem *problems, const qap_
     for (int k = 0; k < count; ++k) {
         const int64_t n = problems[k].n, RT = (int64_t)params[k].R * params[k].T;
         const int64_t ld = hbm_stride((int)n);
-        bytes += 4 * (4 * n * n + 4 * n);            // F, D, Ft, Dt, diagonals, classes, spans
+        bytes += 4 * (4 * n * n + 4 * n) + 4 * n * n;  // F, D, Ft, Dt, 16-bit F and D, diagonals, classes, spans
         bytes += (wide_instance(problems[k]) ? 8 : 4) * RT * n * ld;  // force matrices
         bytes += 4 * 3 * RT * n;                      // perm, pinv, best_perm
         bytes += RT * (8 * 5 + 4 * 3);                // E, bestE, accepted, 2 x cells, pos, order, rep_inst

@@ -62,6 +62,7 @@ struct qap_pt_pool {
     qap::InstDesc *d_inst = nullptr;
     qap::InstState *d_state = nullptr;
     int32_t *d_rep_inst = nullptr, *d_F = nullptr, *d_D = nullptr, *d_Ft = nullptr, *d_Dt = nullptr;
+    int16_t *d_F16 = nullptr, *d_D16 = nullptr;  // 16-bit copies of F and D
     int32_t *d_Fdiag = nullptr, *d_Ddiag = nullptr, *d_fclass = nullptr, *d_fspan = nullptr;
     float *d_beta2f = nullptr;
     double *d_c20 = nullptr;

@@ -105,6 +105,8 @@ int slots_create(int32_t slots, int32_t n_cap, int32_t r_cap, int32_t t_cap, int
     SLOT_TRY(dalloc(&p->d_rep_inst, R));
     SLOT_TRY(dalloc(&p->d_F, p->total_mat));
     SLOT_TRY(dalloc(&p->d_D, p->total_mat));
+    SLOT_TRY(dalloc(&p->d_F16, p->total_mat));
+    SLOT_TRY(dalloc(&p->d_D16, p->total_mat));
     SLOT_TRY(dalloc(&p->d_Ft, p->total_mat));
     SLOT_TRY(dalloc(&p->d_Dt, p->total_mat));
     SLOT_TRY(dalloc(&p->d_Fdiag, p->total_vec));
@@ -208,6 +210,8 @@ int slots_load(qap_pt_pool *p, int slot, const qap_problem &pb, const qap_pt_par
     const int64_t nn = (int64_t)n * n;
     QAP_HIP_TRY(up(p->d_F + d.mat_off, h.F.data(), 4 * nn));
     QAP_HIP_TRY(up(p->d_D + d.mat_off, h.D.data(), 4 * nn));
+    QAP_HIP_TRY(launch_narrow16(p->d_F + d.mat_off, p->d_F16 + d.mat_off, nn, s));
+    QAP_HIP_TRY(launch_narrow16(p->d_D + d.mat_off, p->d_D16 + d.mat_off, nn, s));
     QAP_HIP_TRY(up(p->d_Ft + d.mat_off, h.Ft.data(), 4 * nn));
     QAP_HIP_TRY(up(p->d_Dt + d.mat_off, h.Dt.data(), 4 * nn));
     QAP_HIP_TRY(up(p->d_Fdiag + d.vec_off, h.Fd.data(), 4 * n));

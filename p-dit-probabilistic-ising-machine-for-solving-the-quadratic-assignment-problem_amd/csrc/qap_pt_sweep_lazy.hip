@@ -220,8 +220,10 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
     int16_t *U_s = reinterpret_cast<int16_t *>(red_s + kLazyRed);  // [n4][kpad]
     int16_t *V_s = U_s + (size_t)n4 * kpad;                         // [n4][kpad]
 
-    const int32_t *__restrict__ F = P.F + d.mat_off;
-    const int32_t *__restrict__ D = P.D + d.mat_off;
+    // the 16-bit copies: every entry of an instance placed here fits (the host checks that
+    // 2 max|D| and 4 max|F| do), and half the bytes keep more of the rows in L2
+    const int16_t *__restrict__ F = P.F16 + d.mat_off;
+    const int16_t *__restrict__ D = P.D16 + d.mat_off;
     int32_t *pg = P.perm + d.p_base + (int64_t)local * n;
     int32_t *pinvg = P.pinv + d.p_base + (int64_t)local * n;
     int32_t *bestpg = P.best_perm + d.p_base + (int64_t)local * n;
