@@ -6,8 +6,9 @@
 // wait in LDS as 16-bit vectors stored k-contiguous per row: U[x][k], V[c][k].  Every
 // entry the update rule reads (CPU_PIM.cpp:220-232) is I_base + sum_k U[x][k] V[c][k],
 // formed as 16-bit dot products:
-//   row i      I[i][b] for lane b:  rowI[b] + dot(U[i], V[b])         (rowI = I_base row i)
-//   column a   I[j][a] for lane b:  colA[j] + dot(U[j], V[a])         (colA[x] = I_base[x][a])
+//   row i      I[i][b] for lane b:  I_base[i][b] + dot(U[i], V[b])
+//   column a   I[j][a] for lane b:  I_base[j][a] + dot(U[j], V[a])     (j = pinv(b))
+// (the I_base entries prefetched into lane b's registers one site ahead),
 //   diagonal   dg[c] = I[pinv(c)][c], kept exact move by move; I[i][a] = dg[a].
 // When K reaches the capacity the LDS holds (kmax <= 32, up to 64 from n = 384) the
 // pending updates are applied to I_base in one pass over the matrix, so kmax accepted moves
@@ -39,8 +40,8 @@ constexpr int kLazyMaxK = 64;
 // budget), so those instances run with 512 threads.
 constexpr int kLazyLargeN = 384;
 __host__ __device__ constexpr int lazy_cap(int n) { return n >= kLazyLargeN ? kLazyMaxK : 32; }
-// p, pinv, cls, rnd, dg, drow x2, frow x2, rowI x2, colA x2 -- round8(n) words each
-constexpr int kLazyArrays = 13;
+// p, pinv, cls, rnd, dg, drow x2 -- round8(n) words each
+constexpr int kLazyArrays = 7;
 
 __host__ __device__ inline int lazy_fixed_words(int n) { return kLazyArrays * round8(n) + kLazyRed; }
 
@@ -212,11 +213,8 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
     int32_t *cls_s = pinv_s + n4;
     uint32_t *rnd_s = reinterpret_cast<uint32_t *>(cls_s + n4);
     int32_t *dg_s = reinterpret_cast<int32_t *>(rnd_s + n4);
-    int32_t *drow_s = dg_s + n4;      // 2 buffers
-    int32_t *frow_s = drow_s + 2 * n4;  // 2 buffers
-    int32_t *rowI_s = frow_s + 2 * n4;  // 2 buffers
-    int32_t *colA_s = rowI_s + 2 * n4;  // 2 buffers
-    int32_t *red_s = colA_s + 2 * n4;
+    int32_t *drow_s = dg_s + n4;  // 2 buffers: D rows of this site and the next, for u
+    int32_t *red_s = drow_s + 2 * n4;
     int16_t *U_s = reinterpret_cast<int16_t *>(red_s + kLazyRed);  // [n4][kpad]
     int16_t *V_s = U_s + (size_t)n4 * kpad;                         // [n4][kpad]
 
@@ -247,9 +245,6 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
     for (int x = tid; x < n; x += NT) {
         dg_s[x] = Ig[(int64_t)pinv_s[x] * ld + x];
         drow_s[x] = D[x];
-        frow_s[x] = F[(int64_t)a_cur * n + x];
-        rowI_s[x] = Ig[x];
-        colA_s[x] = Ig[(int64_t)x * ld + a_cur];
     }
     lds_sync<NW>();
 
@@ -270,7 +265,27 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
         pv[q] = (b < n) ? pinv_s[b] : 0;
     }
     int buf = 0, K = 0, a_reg = a_cur;
-    bool stale = false;  // a flush made the prefetched I_base row / column out of date
+    bool stale = false;  // the site's I_base entries in registers are out of date
+
+    // The entries of a site that candidate b = tid + q NT reads go straight to its lane's
+    // registers: I_base[site][b], I_base[pinv(b)][fac], D[site][pinv(b)] and F[fac][b]
+    // (fac = p(site)); the next site's arrive while this site's pick runs.  A move changes
+    // pinv at two candidates only, a and b, whose prefetched pinv-indexed entries then
+    // trade places.
+    auto load_site = [&](int site, int fac, int32_t(&r)[CPL], int32_t(&c)[CPL],
+                         int32_t(&dd)[CPL], int32_t(&f)[CPL]) {
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            const int bc = min(tid + q * NT, n - 1);
+            const int jq = pv[q];
+            r[q] = Ig[(int64_t)site * ld + bc];
+            c[q] = Ig[(int64_t)jq * ld + fac];
+            dd[q] = D[(int64_t)site * n + jq];
+            f[q] = F[(int64_t)fac * n + bc];
+        }
+    };
+    int32_t cr[CPL], cc[CPL], cd[CPL], cf[CPL];  // this site's
+    load_site(0, a_cur, cr, cc, cd, cf);
 
     // I_base += U V^T over the whole matrix on the matrix cores, then no pending update is
     // left.  With helpers the rows are split: U, V and K go to global memory, a request
@@ -331,20 +346,12 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
         for (int i = 0; i < n; ++i) {
             const int a = a_reg;
             const uint32_t ri = rnd_s[i];
-            if (a != a_cur || stale) {  // the prefetched flow row / I_base data is not for this site
+            if (a != a_cur || stale) {  // the last move put another facility here: reload
                 a_cur = a;
-                for (int x = tid; x < n; x += NT) {
-                    frow_s[buf * n4 + x] = F[(int64_t)a * n + x];
-                    rowI_s[buf * n4 + x] = Ig[(int64_t)i * ld + x];
-                    colA_s[buf * n4 + x] = Ig[(int64_t)x * ld + a];
-                }
+                load_site(i, a, cr, cc, cd, cf);
                 stale = false;
-                __syncthreads();
             }
-            const int32_t *drow = drow_s + buf * n4;
-            const int32_t *frow = frow_s + buf * n4;  // F[a][.]
-            const int32_t *rowI = rowI_s + buf * n4;  // I_base[i][.]
-            const int32_t *colA = colA_s + buf * n4;  // I_base[.][a]
+            const int32_t *drow = drow_s + buf * n4;  // D[i][.]
             const int groups = K / 8 + ((K & 7) != 0);
             const v4h *Ui = reinterpret_cast<const v4h *>(U_s + (size_t)i * kpad);
             const v4h *Va = reinterpret_cast<const v4h *>(V_s + (size_t)a * kpad);
@@ -362,7 +369,7 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
                 cv[q] = 0;
                 if (b < n) {
                     const int j = pv[q];
-                    int32_t r = rowI[b], c = colA[j];
+                    int32_t r = cr[q], c = cc[q];
                     const v4h *Vb = reinterpret_cast<const v4h *>(V_s + (size_t)b * kpad);
                     const v4h *Uj = reinterpret_cast<const v4h *>(U_s + (size_t)j * kpad);
                     for (int gg = 0; gg < groups; ++gg) {
@@ -371,24 +378,19 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
                     }
                     rv[q] = r;  // I[i][b]
                     cv[q] = c;  // I[j][a]
-                    const int32_t v = r + c - dg_s[b] - 4 * drow[j] * frow[b];
+                    const int32_t v = r + c - dg_s[b] - 4 * cd[q] * cf[q];
                     dE[q] = v;
                     lmax = max(lmax, v);
                 }
             }
-            // prefetch for the next site: distance row i+1, flow row p(i+1), I_base row
-            // i+1 and I_base column p(i+1); stored at the end of this update
+            // prefetch for the next site: its distance row (to LDS, for u) first, then
+            // the entries its candidates read (registers)
             const int inext = (i + 1 == n) ? 0 : i + 1;
             const int a_nx = p_s[inext];
-            int32_t dn[CPL], frn[CPL], rin[CPL], can[CPL];
+            int32_t dn[CPL], nr[CPL], nc[CPL], nd[CPL], nf[CPL];
 #pragma unroll
-            for (int q = 0; q < CPL; ++q) {
-                const int bc = min(tid + q * NT, n - 1);
-                dn[q] = D[(int64_t)inext * n + bc];
-                frn[q] = F[(int64_t)a_nx * n + bc];
-                rin[q] = Ig[(int64_t)inext * ld + bc];
-                can[q] = Ig[(int64_t)bc * ld + a_nx];
-            }
+            for (int q = 0; q < CPL; ++q) dn[q] = D[(int64_t)inext * n + min(tid + q * NT, n - 1)];
+            load_site(inext, a_nx, nr, nc, nd, nf);
 
             // (2)-(3) heat-bath pick; the winner commits the permutation swap
             const Pick pk =
@@ -406,18 +408,10 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
             }
             {
                 int32_t *dnb = drow_s + (buf ^ 1) * n4;
-                int32_t *fnb = frow_s + (buf ^ 1) * n4;
-                int32_t *rnb = rowI_s + (buf ^ 1) * n4;
-                int32_t *cnb = colA_s + (buf ^ 1) * n4;
 #pragma unroll
                 for (int q = 0; q < CPL; ++q) {
                     const int bb = tid + q * NT;
-                    if (bb < n) {
-                        dnb[bb] = dn[q];
-                        fnb[bb] = frn[q];
-                        rnb[bb] = rin[q];
-                        cnb[bb] = can[q];
-                    }
+                    if (bb < n) dnb[bb] = dn[q];
                 }
             }
             if (pk.owner) {
@@ -435,10 +429,33 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
             lds_sync<NW>();
             const int b = red_s[96], j = red_s[98];
             const int32_t dsel = red_s[97], cv_w = red_s[99];
+            // candidates a and b trade inverse-permutation entries, and so their prefetched
+            // pinv-indexed entries of the next site (handed over in LDS, read after the
+            // next barrier)
+            auto trade_prefetch = [&]() {
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const int bb = tid + q * NT;
+                    if (bb == a) {
+                        nc[q] = red_s[106];
+                        nd[q] = red_s[107];
+                    } else if (bb == b) {
+                        nc[q] = red_s[104];
+                        nd[q] = red_s[105];
+                    }
+                }
+            };
             if (b != a) {
 #pragma unroll
                 for (int q = 0; q < CPL; ++q) {
                     const int bb = tid + q * NT;
+                    if (bb == a) {
+                        red_s[104] = nc[q];
+                        red_s[105] = nd[q];
+                    } else if (bb == b) {
+                        red_s[106] = nc[q];
+                        red_s[107] = nd[q];
+                    }
                     if (bb == b) pv[q] = i;
                     else if (bb == a) pv[q] = j;
                 }
@@ -453,7 +470,7 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
                         const int cc = min(c, n - 1);
                         const int32_t fb = F[(int64_t)b * n + cc];
                         const int32_t dj = D[(int64_t)j * n + cc];
-                        const int32_t v = 2 * (fb - frow[cc]);   // 2 (F[b][c] - F[a][c])
+                        const int32_t v = 2 * (fb - cf[q]);   // 2 (F[b][c] - F[a][c])
                         if (c < n) {
                             U_s[(size_t)c * kpad + K] = (int16_t)(dj - drow[c]);  // D[j][c] - D[i][c]
                             V_s[(size_t)c * kpad + K] = (int16_t)v;
@@ -462,6 +479,7 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
                     }
                     cells += (uint64_t)n * cnt;
                     lds_sync<NW>();
+                    trade_prefetch();
                     // diagonal entries after the move: the swapped facilities take the
                     // entries at their new locations, the others add u[pinv(c)] v[c]
 #pragma unroll
@@ -479,12 +497,9 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
                     // another lane reads (dg[a] of the next site) waits for the pick's
                     // first barrier
                     if (K == kmax) {
-                        // the next site's prefetched I_base row and column take the
-                        // pending updates the flush applies, instead of being read again:
-                        // row[x] += U[inext] . V[x], col[x] += U[x] . V[a_nx]; each thread
-                        // corrects the entries it stored
-                        int32_t *rnb = rowI_s + (buf ^ 1) * n4;
-                        int32_t *cnb = colA_s + (buf ^ 1) * n4;
+                        // the next site's prefetched I_base entries take the pending
+                        // updates the flush applies, instead of being read again:
+                        // I[inext][x] += U[inext] . V[x], I[pinv(x)][a_nx] += U[pinv(x)] . V[a_nx]
                         const v4h *Un = reinterpret_cast<const v4h *>(U_s + (size_t)inext * kpad);
                         const v4h *Vn = reinterpret_cast<const v4h *>(V_s + (size_t)a_nx * kpad);
                         const int kg = K / 8 + ((K & 7) != 0);
@@ -492,33 +507,43 @@ __global__ __launch_bounds__(NT, kLazyWaves<NT>) void sweep_lazy_kernel(PoolDev 
                         for (int q = 0; q < CPL; ++q) {
                             const int x = tid + q * NT;
                             if (x < n) {
-                                int32_t r = rnb[x], c = cnb[x];
+                                int32_t r = nr[q], c = nc[q];
                                 const v4h *Vx = reinterpret_cast<const v4h *>(V_s + (size_t)x * kpad);
-                                const v4h *Ux = reinterpret_cast<const v4h *>(U_s + (size_t)x * kpad);
+                                const v4h *Uj = reinterpret_cast<const v4h *>(U_s + (size_t)pv[q] * kpad);
                                 for (int gg = 0; gg < kg; ++gg) {
                                     r = dot8(Un[gg], Vx[gg], r);
-                                    c = dot8(Ux[gg], Vn[gg], c);
+                                    c = dot8(Uj[gg], Vn[gg], c);
                                 }
-                                rnb[x] = r;
-                                cnb[x] = c;
+                                nr[q] = r;
+                                nc[q] = c;
                             }
                         }
                         flush(true);
                     }
-                } else if (tid == 0) {
+                } else {
                     // identical flow rows and columns: I's columns a and b are equal, so the
                     // diagonal entries of the two facilities trade places
-                    const int32_t t = dg_s[a];
-                    dg_s[a] = dg_s[b];
-                    dg_s[b] = t;
+                    if (tid == 0) {
+                        const int32_t t = dg_s[a];
+                        dg_s[a] = dg_s[b];
+                        dg_s[b] = t;
+                    }
+                    lds_sync<NW>();
+                    trade_prefetch();
                 }
-                if (((cls_s[a] ^ cls_s[b]) & 0xffff) == 0) lds_sync<NW>();
                 if (E < bestE) {  // CPU_PIM.cpp:271-272, plus the permutation
                     bestE = E;
                     for (int x = tid; x < n; x += NT) bestpg[x] = p_s[x];
                 }
             }
             buf ^= 1;
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) {  // the next site's entries become current
+                cr[q] = nr[q];
+                cc[q] = nc[q];
+                cd[q] = nd[q];
+                cf[q] = nf[q];
+            }
             a_cur = a_nx;
             a_reg = (b != a && j == inext) ? a : a_nx;
         }
