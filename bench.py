@@ -176,9 +176,11 @@ def roofline_pass(pool, insts, sweeps, traffic_name):
     tfile = os.path.join(ROOT, "profiles", f"traffic_{traffic_name}.txt")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            rec = json.load(fh)
-        if rec.get("shape") == best["shape"] and "hbm_bytes_per_interval" in rec:
-            traffic = rec["hbm_bytes_per_interval"] * intervals_per_launch
+            recs = json.load(fh)
+        # one record, or one per kernel shape of the workload ("records")
+        for rec in recs.get("records", [recs]):
+            if rec.get("shape") == best["shape"] and "hbm_bytes_per_interval" in rec:
+                traffic = rec["hbm_bytes_per_interval"] * intervals_per_launch
     out = {
         "bound": "lds" if lds else "hbm",
         "achieved": achieved,
@@ -252,19 +254,23 @@ def cpu_rate_samples(insts, sample, sweeps, threads, repeats=5):
     return rates[len(rates) // 2], rates
 
 
-def cpu_baseline_single(inst, sweeps, threads, repeats=5):
+def cpu_baseline_single(inst, sweeps, threads, repeats=5, rung_step=1):
     """--mode throughput: `threads` independent ladders of the workload's instance (the
     reference's -t model), `sweeps` sweeps each, oracle reference arithmetic; median of
-    `repeats` runs."""
+    `repeats` runs.  rung_step > 1 keeps every rung_step-th rung of the ladder (the same
+    temperature range with fewer replicas) to bound the sample."""
     po = _oracle()
+    betas = inst.beta_ln[::rung_step]
     rates, att, dt = [], 0, 0.0
-    for _ in range(repeats):
+    for k in range(repeats):
         t0 = time.perf_counter()
-        r = po.solve(inst.F, inst.D, inst.beta_ln, sweeps, T=threads, seed=inst.seed,
+        r = po.solve(inst.F, inst.D, betas, sweeps, T=threads, seed=inst.seed,
                      mode=po.MODE_REFFP, nthreads=threads)
         dt = time.perf_counter() - t0
         att = r["attempts"]
         rates.append(att / dt)
+        print(f"# cpu baseline run {k + 1}/{repeats}: {att} attempts in {dt:.1f} s",
+              file=sys.stderr, flush=True)
     rates.sort()
     return rates[len(rates) // 2], rates, att, dt
 
@@ -485,12 +491,19 @@ def throughput(pkg, args):
         host = host_info()
         threads = args.cpu_threads or max(1, min(16, host["nproc"]))
         sweeps = {"grey256": 1, "flat64": 400, "large512": 1, "large1024": 1}[args.workload]
-        med, rates, att, dt = cpu_baseline_single(insts[0], sweeps, threads)
+        # runs of the sample: one ladder-sweep per thread is ~12 s at n = 512 in this
+        # arithmetic (a full n^2 force update per accepted move)
+        repeats = args.cpu_repeats or {"large512": 3, "large1024": 3}.get(args.workload, 5)
+        # n = 1024: every 8th rung (8 replicas over the same temperature range), ~25 s a run
+        rung_step = 8 if args.workload == "large1024" else 1
+        med, rates, att, dt = cpu_baseline_single(insts[0], sweeps, threads, repeats, rung_step)
+        reps = len(insts[0].beta_ln[::rung_step])
         out["cpu_baseline"] = {
             "value": med, "unit": "swap-attempts/s", "cores": threads, "kind": "port",
-            "sample": (f"{threads} independent ladders x {insts[0].R} replicas x {sweeps} "
+            "sample": (f"{threads} independent ladders x {reps} replicas"
+                       f"{' (every 8th rung of the ladder)' if rung_step > 1 else ''} x {sweeps} "
                        f"sweeps of n={insts[0].n} on {threads} threads ({att} attempts, "
-                       f"{dt:.1f} s per run), median of 5 runs; oracle reference-arithmetic "
+                       f"{dt:.1f} s per run), median of {len(rates)} runs; oracle reference-arithmetic "
                        f"mode (CPU_PIM.cpp:195-290)"),
             "runs": rates, "host": host}
     return out
@@ -609,6 +622,7 @@ def served(pkg, args):
         best_by_rate.append(best)
         print(f"# served rate {rate}/s: {json.dumps(row)}", flush=True)
     nplain = min(N, args.plain_sample)
+    print(f"# plain stream: {nplain} requests admitted one at a time", flush=True)
     plain, best_p = stream(float(args.rates.split(",")[-1]), reqs[:nplain], 1, 1, 0.0)
     same = all(b[:nplain] == best_p for b in best_by_rate) and \
         all(b == best_by_rate[0] for b in best_by_rate)
@@ -657,6 +671,9 @@ def main():
                          "counts as a sweep of n attempts")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-repeats", type=int, default=0,
+                    help="runs of the --mode throughput CPU baseline (default 5; 3 at n = 512 "
+                         "and 1024)")
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--tts-sweeps", type=int, default=10000)
     ap.add_argument("--tts-chunks", type=int, default=50)
