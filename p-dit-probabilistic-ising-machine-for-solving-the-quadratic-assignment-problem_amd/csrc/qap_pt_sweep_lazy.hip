@@ -35,7 +35,7 @@ constexpr int kLazyRed = 160;  // reduction slots (same use as the other sweep k
 constexpr int kLazyMaxK = 64;
 // Below n = 384 at most 32 pending updates: the LDS of 32 leaves room for more workgroups
 // per CU, which the full-chip loads use.  From n = 384 on one workgroup holds its CU's LDS
-// anyway, and up to 64 pending updates (56 fit at n = 512) halve the flushes per sweep;
+// anyway, and up to 64 pending updates (all 64 fit at n = 512) halve the flushes per sweep;
 // only the 512-thread shapes take more than 32 (the smaller ones keep their register
 // budget), so those instances run with 512 threads.
 constexpr int kLazyLargeN = 384;
