@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 def _same(a, b):
     for key in ("best_cost", "attempts", "accepted", "exchanges", "kicks", "update_cells",
-                "sweeps_done"):
+                "flush_cells", "sweeps_done"):
         assert a[key] == b[key], (key, a[key], b[key])
     np.testing.assert_array_equal(a["best_perm"], b["best_perm"])
 
