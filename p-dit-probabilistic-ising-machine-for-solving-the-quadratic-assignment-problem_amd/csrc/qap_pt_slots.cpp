@@ -22,7 +22,7 @@
 namespace qap {
 namespace {
 
-constexpr int kBankLazyPending = 24;  // pending rank-2 updates a lazy bank slot holds
+constexpr int kBankLazyPending = 32;  // pending rank-2 updates a lazy bank slot holds
 
 struct SlotLayout {
     int64_t mat, vec, beta, p, I, reps, ladders;
@@ -274,11 +274,11 @@ int slots_launch(qap_pt_pool *p, int intervals, hipEvent_t done) {
             }
             Bucket &B = p->buckets[b];
             B.shape.sparse = B.shape.sparse && d.sparse_upd != 0;  // all-sparse buckets only
-            // lazy slots hold 24 pending updates instead of up to 32: the smaller LDS
-            // footprint lets more workgroups share a CU (n <= 256: three instead of two),
-            // which at a bank's load outweighs the extra flushes (served stream 245 -> 264 M
-            // attempts/s; a standalone 512-instance pool measured 12 % slower with 24 and
-            // keeps 32)
+            // lazy slots hold up to kBankLazyPending pending updates.  24 instead of 32 let
+            // a third workgroup share a CU at n <= 256 while the kernel kept six rows of the
+            // site in LDS (served stream 245 -> 264 M attempts/s); with those rows in
+            // registers 32 fit three workgroups there as well, and 32 measured 266-270 ->
+            // 282 M on the served stream)
             const int lds = sh.im == IM_LAZY ? std::min(sweep_lds_bytes(d.n, d.ld_s, sh.im),
                                                         lazy_lds_bytes(d.n, kBankLazyPending))
                                              : sweep_lds_bytes(d.n, d.ld_s, sh.im);
