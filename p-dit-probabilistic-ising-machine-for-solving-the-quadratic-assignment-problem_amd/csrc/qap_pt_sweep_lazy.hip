@@ -31,7 +31,10 @@ namespace qap {
 
 namespace {
 
-constexpr int kLazyRed = 160;  // reduction slots (same use as the other sweep kernel)
+// reduction slots: [0,16) per-wave maxima and [16,80) per-wave prefix sums of the pick,
+// [96,100) the winner (facility, delta, location, I[j][a]), [104,108) the next site's
+// p^-1-indexed entries two candidates trade on a move, [140,156) per-wave update counts
+constexpr int kLazyRed = 160;
 constexpr int kLazyMaxK = 64;
 // Below n = 384 at most 32 pending updates: the LDS of 32 leaves room for more workgroups
 // per CU, which the full-chip loads use.  From n = 384 on one workgroup holds its CU's LDS
