@@ -613,22 +613,61 @@ def served(pkg, args):
         att = sum(r["attempts"] for r in res)
         return {"rate_per_s": rate, "requests": len(reqs), "attempts_per_s": att / wall,
                 "latency_p50_s": _pct(lat, 50), "latency_p99_s": _pct(lat, 99),
-                "wall_s": wall}, [r["best_cost"] for r in res]
+                "wall_s": wall}, [r["best_cost"] for r in res], lat
 
     rows, best_by_rate = [], []
     for rate in [float(x) for x in args.rates.split(",")]:
-        row, best = stream(rate, reqs, args.max_pools, args.max_batch, args.batch_window)
+        row, best, _ = stream(rate, reqs, args.max_pools, args.max_batch, args.batch_window)
         rows.append(row)
         best_by_rate.append(best)
         print(f"# served rate {rate}/s: {json.dumps(row)}", flush=True)
     nplain = min(N, args.plain_sample)
     print(f"# plain stream: {nplain} requests admitted one at a time", flush=True)
-    plain, best_p = stream(float(args.rates.split(",")[-1]), reqs[:nplain], 1, 1, 0.0)
+    plain, best_p, _ = stream(float(args.rates.split(",")[-1]), reqs[:nplain], 1, 1, 0.0)
     same = all(b[:nplain] == best_p for b in best_by_rate) and \
         all(b == best_by_rate[0] for b in best_by_rate)
     top = max(r["attempts_per_s"] for r in rows)
+
+    # time-to-target: a sample of the requests with an equal-sweeps budget, each carrying a
+    # goal of 1.01 x the best of the oracle's reference-arithmetic run (CPU_PIM.cpp:195-290)
+    # of the same instance, ladder and seed at those sweeps; a request ends when its ladder
+    # best reaches the goal (CPU_PIM.cpp:346-348) or its budget runs out, so its latency
+    # from submit entry is its time to target.  The same requests one at a time give the
+    # plain path's.
+    tts = None
+    if not args.no_cpu:
+        host = host_info()
+        threads = args.cpu_threads or max(1, min(16, host["nproc"]))
+        ns, S = min(N, args.served_tts_sample), args.served_tts_sweeps
+        base = [pkg.Instance(F=r.F, D=r.D, beta_ln=r.beta_ln, sweeps=S, seed=r.seed)
+                for r in reqs[:ns]]
+        print(f"# served time-to-target: reference-arithmetic runs of {ns} requests at {S} "
+              f"sweeps on {threads} threads", flush=True)
+        ref_rows, cpu_wall = cpu_reference_runs(base, list(range(ns)), S, threads)
+        goal = {k: int(b) + abs(int(b)) * 10 // 1000 for k, b, _, _ in ref_rows}
+        greqs = [pkg.Instance(F=x.F, D=x.D, beta_ln=x.beta_ln, sweeps=S, seed=x.seed,
+                              goal=goal[k]) for k, x in enumerate(base)]
+        rate_hi = float(args.rates.split(",")[-1])
+        trow, tbest, tlat = stream(rate_hi, greqs, args.max_pools, args.max_batch,
+                                   args.batch_window)
+        print(f"# served time-to-target: {json.dumps(trow)}", flush=True)
+        prow, pbest, plat = stream(rate_hi, greqs, 1, 1, 0.0)
+
+        def reached(best, lat):
+            hit = [t for k, (b, t) in enumerate(zip(best, lat)) if b <= goal[k]]
+            return {"p50_s": _pct(hit, 50), "p99_s": _pct(hit, 99), "reached": len(hit)}
+
+        tts = {"target": "best cost <= 1.01 x the reference best",
+               "target_source": (f"best cost of the oracle's reference-arithmetic run "
+                                 f"(CPU_PIM.cpp:195-290 roulette) at {S} sweeps, per request"),
+               "requests": ns, "sweeps": S, "rate_per_s": rate_hi,
+               "clock": "submit entry to completion; a request completes when its goal is "
+                        "reached or its sweeps run out",
+               "served": reached(tbest, tlat), "plain": reached(pbest, plat),
+               "results_identical": tbest == pbest,
+               "target_runs": {"wall_s": cpu_wall, "threads": threads}}
     return {
-        "metric": "swap-attempts/s; p50/p99 time-to-solution (served vs plain)",
+        "metric": "swap-attempts/s; p50/p99 time-to-solution and time-to-target (served vs plain)",
         "value": top,
         "unit": "swap-attempts/s",
         "n_gpus": 1,
@@ -645,6 +684,7 @@ def served(pkg, args):
         "served_over_plain": top / plain["attempts_per_s"],
         "results_identical_across_rates_and_plain": bool(same),
         "latency": "submit to completion of the full sweep budget",
+        "time_to_target": tts,
     }
 
 
@@ -679,6 +719,10 @@ def main():
     ap.add_argument("--tts-chunks", type=int, default=50)
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--served-sweeps", type=int, default=10000)
+    ap.add_argument("--served-tts-sweeps", type=int, default=20,
+                    help="sweep budget of the served time-to-target requests (the reference-"
+                         "arithmetic runs that set their goals run as many)")
+    ap.add_argument("--served-tts-sample", type=int, default=32)
     ap.add_argument("--rates", default="10,100,1000")
     ap.add_argument("--max-pools", type=int, default=16)
     ap.add_argument("--max-batch", type=int, default=64)
