@@ -723,7 +723,7 @@ def main():
                     help="sweep budget of the served time-to-target requests (the reference-"
                          "arithmetic runs that set their goals run as many)")
     ap.add_argument("--served-tts-sample", type=int, default=32)
-    ap.add_argument("--rates", default="10,100,1000")
+    ap.add_argument("--rates", default="2,10,100,1000")
     ap.add_argument("--max-pools", type=int, default=16)
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--batch-window", type=float, default=0.002)
