@@ -105,6 +105,8 @@ struct InstDesc {
     int32_t kick_after;
     int32_t has_goal;
     int32_t sym;              // F and D symmetric with zero diagonal
+    int32_t xmirror;          // persistent launches mirror the exchange step in every
+                              //   workgroup (R*T <= 64, every cost below 2^31 - 1)
     int64_t goal;
     int64_t mat_off;          // F/D/Ft/Dt element offset
     int64_t vec_off;          // Fdiag/Ddiag element offset (n)
@@ -157,6 +159,9 @@ struct PoolDev {
     uint32_t *sync;           // persistent sweep launches: [0] fault word (a wait that gave
                               //   up), then per instance [1 + 2 k] arrivals and [2 + 2 k] the
                               //   number of exchange steps run inside persistent launches
+    uint64_t *xslot;          // persistent launches with mirrored exchange steps: per replica
+                              //   two parities x {cost, best}, each tag << 32 | value
+                              //   (null where the pool has none)
 };
 
 }  // namespace qap

@@ -13,6 +13,10 @@ namespace qap {
 
 // LDS available to one workgroup on gfx950 (160 KiB per CU).
 constexpr int kMaxLdsBytes = 160 * 1024;
+// sites per round of the speculative sweep kernel in latency-bound pools that admit it
+// (eight waves: two per SIMD; sixteen lose more to the shared LDS and barrier than the
+// extra sites gain on the n = 64, 16-replica pool)
+constexpr int kSpecSites = 8;
 
 // Lets kernel K take up to kMaxLdsBytes of dynamic LDS.  Set once per process and kernel
 // instantiation; callers on several threads (a pool's owner, the serving worker) are
@@ -51,11 +55,16 @@ struct SweepShape {
     // every instance of the launch has sparse flow: the kernel built without the dense
     // update path runs (set per bucket; any instance is still updated exactly there)
     bool sparse = false;
+    // > 0: spec sites evaluated at once, one wave each (sweep_spec_kernel: symmetric,
+    // n <= 64, int32 LDS), the first moved one committed; 0: one site at a time
+    int spec = 0;
 };
 
 // whether a kernel instance exists for nt threads x cpl candidates under placement im
 bool sweep_shape_supported(int nt, int cpl, int im);
 int sweep_lds_bytes(int n, int ld_s, int im);
+// LDS of a workgroup of shape sh for size n (the speculative kernel's layout when sh.spec)
+int shape_lds_bytes(int n, int ld_s, const SweepShape &sh);
 // LDS of the lazy kernel for size n, and whether it holds at least 8 pending updates
 int lazy_lds_bytes(int n);
 // LDS of a lazy workgroup for size n holding at most max_pending (a multiple of 8) updates

@@ -244,6 +244,16 @@ SweepShape choose_shape(int n, bool sym, bool nonneg, int64_t force_bound, bool 
     }
     *ld_s = lds_stride(n, s.im);
     shape_for_size(n, sparse, s, !throughput);
+    // A latency-bound pool of symmetric instances that fit one wave in int32 LDS evaluates
+    // several sites at once, a wave each (sweep_spec_kernel): a round costs about what one
+    // site does, and advances to the first site whose update moved a facility, so mostly
+    // rejected updates (cold replicas) run up to that many times faster.
+    // QAP_PT_SPEC=0|2|4|8|16 sets the sites per round (0: one at a time).
+    if (!throughput && s.im == IM_LDS32 && sym && n <= 64) {
+        int sp = kSpecSites;
+        if (const char *e = std::getenv("QAP_PT_SPEC")) sp = std::atoi(e);
+        s.spec = (sp == 2 || sp == 4 || sp == 8 || sp == 16) ? sp : 0;
+    }
     return s;
 }
 
@@ -353,8 +363,8 @@ void pool_free(qap_pt_pool *p) {
                     p->d_Ft,    p->d_Dt,        p->d_Fdiag,    p->d_Ddiag,  p->d_beta2f,
                     p->d_c20,   p->d_perm,      p->d_pinv,     p->d_best_perm, p->d_I, p->d_I64,
                     p->d_E,     p->d_bestE,     p->d_limits,   p->d_accepted, p->d_cells, p->d_pos,
-                    p->d_order, p->d_tss,       p->d_ladder_best, p->d_sync, p->d_uv,
-                    p->d_fwords};
+                    p->d_order, p->d_tss,       p->d_ladder_best, p->d_sync, p->d_xslot,
+                    p->d_uv,    p->d_fwords};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto st : p->side) (void)hipStreamDestroy(st);
@@ -587,6 +597,16 @@ void prepare_instance(const qap_problem &pb, const qap_pt_params &pr, bool throu
     d.kick_after = pr.kick_after;
     d.has_goal = pr.has_goal ? 1 : 0;
     d.sym = sym ? 1 : 0;
+    // Persistent launches run the exchange step in every workgroup from 32-bit tagged costs
+    // when a ladder set fits one wave and every cost fits: |E| <= max|F| sum|D|.
+    // QAP_PT_XMIRROR=0 keeps the counter protocol (one workgroup runs each step).
+    {
+        __int128 sumD = 0;
+        for (int64_t x = 0; x < nn; ++x) sumD += std::llabs((long long)pb.D[x]);
+        const bool fits = (__int128)bnd.maxF * sumD < (__int128)INT32_MAX;
+        const char *env = std::getenv("QAP_PT_XMIRROR");
+        d.xmirror = (pr.R * pr.T <= 64 && fits && !(env && env[0] == '0')) ? 1 : 0;
+    }
     d.goal = pr.goal;
     d.seed_lo = (uint32_t)pr.seed;
     d.seed_hi = (uint32_t)(pr.seed >> 32);
@@ -715,6 +735,7 @@ void bind_pool(qap_pt_pool *p) {
     P.tss = p->d_tss;
     P.ladder_best = p->d_ladder_best;
     P.sync = p->d_sync;
+    P.xslot = p->d_xslot;
 }
 
 }  // namespace qap
@@ -759,7 +780,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     std::vector<int32_t> hF, hD, hFt, hDt, hFd, hDd, hCls, hSpan, rep_inst, perm, pinv, pos, order;
     std::vector<float> beta2f;
     std::vector<double> c20;
-    std::map<std::tuple<int, int, int, int>, int> bucket_of;
+    std::map<std::tuple<int, int, int, int, int>, int> bucket_of;
     auto append = [](auto &dst, const auto &src) { dst.insert(dst.end(), src.begin(), src.end()); };
     int64_t all_reps = 0;
     for (int k = 0; k < count; ++k) all_reps += (int64_t)params[k].R * params[k].T;
@@ -808,7 +829,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
         append(order, h.pos);
         for (int r = 0; r < RT; ++r) rep_inst.push_back(k);
         const SweepShape &sh = h.shape;
-        const auto key = std::make_tuple(sh.nt, sh.cpl, sh.im, (int)sh.sym);
+        const auto key = std::make_tuple(sh.nt, sh.cpl, sh.im, (int)sh.sym, sh.spec);
         auto it = bucket_of.find(key);
         int bi;
         if (it == bucket_of.end()) {
@@ -822,7 +843,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
         }
         Bucket &B = p->buckets[bi];
         B.shape.sparse = B.shape.sparse && d.sparse_upd != 0;  // all-sparse buckets only
-        B.lds_bytes = std::max(B.lds_bytes, sweep_lds_bytes(n, d.ld_s, sh.im));
+        B.lds_bytes = std::max(B.lds_bytes, shape_lds_bytes(n, d.ld_s, sh));
         for (int r = 0; r < RT; ++r) B.reps.push_back(d.rep_base + r);
     }
 
@@ -906,6 +927,7 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(palloc(p, &p->d_tss, p->total_ladders));
     POOL_TRY(palloc(p, &p->d_ladder_best, p->total_ladders));
     POOL_TRY(palloc(p, &p->d_sync, 1 + 2 * (size_t)count));
+    POOL_TRY(palloc(p, &p->d_xslot, 4 * (size_t)RTall));
     for (auto &b : p->buckets) POOL_TRY(palloc(p, &b.d_list, b.reps.size()));
     if (p->helpers > 0) {  // pending updates and flush words of the helped lazy bucket
         const size_t nb = p->buckets[0].reps.size();
@@ -962,6 +984,8 @@ int pool_create_impl(int32_t count, const qap_problem *problems, const qap_pt_pa
     POOL_TRY(hipMemsetAsync(p->d_I64, 0, sizeof(int64_t) * std::max<int64_t>(p->total_I64, 1),
                             p->stream));
     POOL_TRY(hipMemsetAsync(p->d_sync, 0, sizeof(uint32_t) * (1 + 2 * (size_t)count), p->stream));
+    // tag 0 is never awaited: the first exchange step inside a launch has megas_done >= 1
+    POOL_TRY(hipMemsetAsync(p->d_xslot, 0, sizeof(uint64_t) * 4 * (size_t)RTall, p->stream));
     POOL_TRY(launch_narrow16(p->d_F, p->d_F16, p->total_mat, p->stream));
     POOL_TRY(launch_narrow16(p->d_D, p->d_D16, p->total_mat, p->stream));
 
@@ -1237,6 +1261,7 @@ int64_t pool_device_bytes(int32_t count, const qap_problem *problems, const qap_
         bytes += (wide_instance(problems[k]) ? 8 : 4) * RT * n * ld;  // force matrices
         bytes += 4 * 3 * RT * n;                      // perm, pinv, best_perm
         bytes += RT * (8 * 5 + 4 * 3);                // E, bestE, accepted, 2 x cells, pos, order, rep_inst
+        bytes += RT * 8 * 4;                          // tagged cost words of mirrored exchange steps
         bytes += 24 * params[k].T + 16 * params[k].R + sizeof(InstDesc) + sizeof(InstState) + 8;
     }
     return bytes;
@@ -1404,9 +1429,11 @@ int qap_pt_pool_shape(qap_pt_pool *p, int32_t inst, char *buf, int32_t buflen) {
     if (inst < 0 || inst >= p->count) return fail(QAP_PT_E_INVAL, "instance %d out of range", inst);
     const SweepShape &s = p->shapes[inst];
     const InstDesc &d = p->desc[inst];
-    std::snprintf(buf, buflen, "%dx%d,%s,%s,%s", s.nt, s.cpl,
+    char spec[16] = "";
+    if (s.spec > 0) std::snprintf(spec, sizeof(spec), ",spec%d", s.spec);
+    std::snprintf(buf, buflen, "%dx%d,%s,%s%s,%s", s.nt, s.cpl,
                   placement_name(s.im),
-                  s.sym ? "sym" : "general", d.sparse_upd ? "sparse-flow" : "dense-flow");
+                  s.sym ? "sym" : "general", spec, d.sparse_upd ? "sparse-flow" : "dense-flow");
     return QAP_PT_OK;
 }
 
@@ -1417,10 +1444,13 @@ int qap_pt_pool_bucket(qap_pt_pool *p, int32_t b, char *shape, int32_t buflen,
         return fail(QAP_PT_E_INVAL, "bucket %d out of range (%d buckets)", b,
                     (int)p->buckets.size());
     const Bucket &B = p->buckets[b];
-    if (shape && buflen > 0)
-        std::snprintf(shape, buflen, "%dx%d,%s,%s%s", B.shape.nt, B.shape.cpl,
-                      placement_name(B.shape.im), B.shape.sym ? "sym" : "general",
+    if (shape && buflen > 0) {
+        char spec[16] = "";
+        if (B.shape.spec > 0) std::snprintf(spec, sizeof(spec), ",spec%d", B.shape.spec);
+        std::snprintf(shape, buflen, "%dx%d,%s,%s%s%s", B.shape.nt, B.shape.cpl,
+                      placement_name(B.shape.im), B.shape.sym ? "sym" : "general", spec,
                       B.shape.sparse ? ",sparse-flow" : "");
+    }
     if (replicas) *replicas = (int64_t)B.reps.size();
     if (kernel_ms) *kernel_ms = b < (int32_t)p->bucket_ms.size() ? p->bucket_ms[b] : 0.0;
     if (launches) *launches = b < (int32_t)p->bucket_launches.size() ? p->bucket_launches[b] : 0;

@@ -72,6 +72,7 @@ struct qap_pt_pool {
     uint64_t *d_accepted = nullptr, *d_cells = nullptr;
     int32_t *d_pos = nullptr, *d_order = nullptr, *d_tss = nullptr;
     uint32_t *d_sync = nullptr;  // persistent launches: fault word + per instance 2 words
+    uint64_t *d_xslot = nullptr; // persistent launches: tagged cost words, 4 per replica
     // latency-bound pool of LDS placements: exchange intervals run as persistent
     // cooperative launches (replicas stay in LDS, exchange steps in the kernel)
     bool persistent = false;

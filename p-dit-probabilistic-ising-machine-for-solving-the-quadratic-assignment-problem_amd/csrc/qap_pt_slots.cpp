@@ -175,6 +175,7 @@ int slots_load(qap_pt_pool *p, int slot, const qap_problem &pb, const qap_pt_par
     // stream ran 178 M attempts/s against 241 M with one shape)
     prepare_instance(pb, pr, throughput_regime((int64_t)p->count * p->rt_cap), h);
     if (h.shape.im != IM_ONESHOT) shape_for_size(p->n_cap, h.d.sparse_upd != 0, h.shape);
+    h.shape.spec = 0;  // bank slots share the one-site kernels (their LDS is sized for them)
     const SlotLayout L = layout(p->n_cap, p->r_cap, p->t_cap, p->rt_cap);
     InstDesc &d = h.d;
     const int n = d.n, RT = d.R * d.T;

@@ -33,13 +33,14 @@
 #include "qap_pt_device.h"
 #include "qap_pt_launch.h"
 #include "qap_pt_sweep_common.h"
+#include "qap_pt_xmirror.h"
 
 namespace qap {
 
 // red_s (ints): [0,16) max partials, [16,80) sum partials [q*16 + wave] (in the commit of
 // a sparse update, [16, 16 + 2 NW CPL) holds the uint64 nonzero-column masks instead),
-// [96] winner b, [97] winner delta, [98] winner's partner location j,
-// [140, 140 + 2 NW) per-wave rewritten-cell counts (uint64) at the end of a launch.
+// [96] winner b, [97] winner delta, [98] winner's partner location j, [100, 103) the
+// exchange step's outcome in a persistent launch, [140, 140 + 2 NW) per-wave rewritten-cell counts (uint64) at the end of a launch.
 constexpr int kRedWords = 160;
 // sparse flow updates with at most this many nonzero columns walk the columns directly
 // (one LDS round trip per column, in sequence); longer sets batch rows over a compacted
@@ -222,16 +223,86 @@ struct IElem<IM_LDS16> {
 __device__ void exchange_instance(const PoolDev &P, int inst, int cap, unsigned char *ex_smem,
                                   int tid);
 
-// Longest a persistent workgroup waits for the rest of its ladder at an exchange step:
-// 30 s of the 100 MHz real-time counter.  The launch is cooperative (every workgroup
-// resident), so the wait ends when the slowest replica finishes its interval; the limit
-// only turns a fault into a reported error (the fault word) instead of a hang.
-constexpr uint64_t kPersistWaitTicks = 3000000000ull;
+// The exchange step between two intervals of a persistent launch, called by every thread of
+// replica g's workgroup (local index `local`, cost E, best bestE).  Returns kXmRun with the
+// replica's new ladder position and first sweep in *pos and *sweep0, kXmStop when the
+// instance ends at this step (limit or goal), kXmFault when a wait gave up (the fault word
+// is set).  With `mirror` every workgroup runs the step itself on its copy xm (wave 0);
+// otherwise the last replica to arrive at the instance's counter runs exchange_instance and
+// advances the generation word to `target`, for which the others wait.
+__device__ int persistent_exchange(const PoolDev &P, const InstDesc &d, int inst, int g, int local,
+                                   int64_t E, int64_t bestE, uint32_t target, bool mirror,
+                                   XMirror &xm, int32_t *red_s, int tid, int *pos,
+                                   uint32_t *sweep0) {
+    const int lane = tid & (kWave - 1), wid = tid / kWave;
+    if (mirror) {
+        if (wid == 0) {
+            int np = 0;
+            const int r = xm_exchange(P, d, xm, g, local, E, bestE, lane, &np);
+            if (lane == 0) {
+                red_s[100] = r;
+                red_s[101] = np;
+                red_s[102] = (int)xm.sweep_base;
+            }
+        }
+        __syncthreads();
+        const int r = red_s[100];
+        if (r == kXmRun) {
+            *pos = red_s[101];
+            *sweep0 = (uint32_t)red_s[102];
+        }
+        return r;
+    }
+    if (tid == 0) {  // this replica's cost and best, for the exchange step
+        P.E[g] = E;
+        P.bestE[g] = bestE;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t prev = __hip_atomic_fetch_add(&P.sync[1 + 2 * inst], 1u, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+        red_s[100] = prev == (uint32_t)(d.R * d.T - 1);  // the instance's last arrival
+        red_s[101] = 0;
+    }
+    __syncthreads();
+    if (red_s[100]) {
+        if (wid == 0) {  // every other replica has arrived: run the step, release them
+            if (lane == 0)
+                __hip_atomic_store(&P.sync[1 + 2 * inst], 0u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            exchange_instance(P, inst, 0, nullptr, lane);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0)
+                __hip_atomic_store(&P.sync[2 + 2 * inst], target, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else if (tid == 0) {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        // relaxed polls (an acquire per poll would invalidate the caches other
+        // workgroups read from); the acquire fence after the barrier below follows
+        while (__hip_atomic_load(&P.sync[2 + 2 * inst], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT) != target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (__builtin_amdgcn_s_memrealtime() - t0 > kPersistWaitTicks) {
+                __hip_atomic_store(&P.sync[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                red_s[101] = 1;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (red_s[101]) return kXmFault;
+    const InstState stn = P.state[inst];
+    if (!stn.active) return kXmStop;  // limit or goal reached at this step
+    *pos = P.pos[g];
+    *sweep0 = stn.sweep_base;
+    return kXmRun;
+}
 
 // max_iv > 1 (persistent launch, LDS placements): the workgroup runs up to max_iv exchange
-// intervals with its replica resident in LDS.  Between intervals it publishes its cost and
-// best and arrives at its instance's counter; the last replica of the instance to arrive
-// runs the instance's exchange step (the one exchange_kernel runs between launches) and
+// intervals with its replica resident in LDS.  Between intervals the instance's exchange
+// step (the one exchange_kernel runs between launches) runs inside the launch: mirrored by
+// every workgroup from the replicas' tagged costs (persistent_exchange; instances with
+// InstDesc::xmirror), or run by the last replica to arrive at the instance's counter, which
 // releases the others by advancing the instance's generation word.
 template <int NT, int CPL, int IM, bool SYM, bool SP>
 __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__restrict__ rep_list,
@@ -363,10 +434,15 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     float beta2f = P.beta2f[d.beta_off + pos];
     double c20 = P.c20[d.beta_off + pos];
     uint32_t sweep0 = P.state[inst].sweep_base;
-    // exchange steps of this instance done inside persistent launches so far (read before
-    // this workgroup first arrives, so no step of this launch can have advanced it)
+    // persistent launch: the instance's exchange state mirrored in wave 0, or the exchange
+    // steps of this instance done inside persistent launches so far (read before this
+    // workgroup first arrives, so no step of this launch can have advanced it)
+    const bool mirror = max_iv > 1 && d.xmirror && P.xslot;
+    XMirror xm{};
+    int xstat = kXmRun;
+    if (mirror && wid == 0) xm = xm_load(P, inst, d, lane);
     uint32_t gen0 = 0;
-    if (max_iv > 1)
+    if (max_iv > 1 && !mirror)
         gen0 = __hip_atomic_load(&P.sync[2 + 2 * inst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // SP: the launch holds sparse-flow instances only, and the dense update path is not
     // built (grey n = 256: 2.2 % faster; the sparse path is exact for any flow)
@@ -387,54 +463,12 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     int a_reg = a_cur;
 
     for (int iv = 0; iv < max_iv; ++iv) {
-    if (iv > 0) {
-        // ---- exchange step between intervals (persistent launch) ----------------
-        if (tid == 0) {  // this replica's cost and best, for the exchange step
-            P.E[g] = E;
-            P.bestE[g] = bestE;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            const uint32_t prev = __hip_atomic_fetch_add(&P.sync[1 + 2 * inst], 1u, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-            red_s[100] = prev == (uint32_t)(d.R * d.T - 1);  // the instance's last arrival
-            red_s[101] = 0;
-        }
-        __syncthreads();
-        const uint32_t target = gen0 + (uint32_t)iv;
-        if (red_s[100]) {
-            if (wid == 0) {  // every other replica has arrived: run the step, release them
-                if (lane == 0)
-                    __hip_atomic_store(&P.sync[1 + 2 * inst], 0u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                exchange_instance(P, inst, 0, nullptr, lane);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                if (lane == 0)
-                    __hip_atomic_store(&P.sync[2 + 2 * inst], target, __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            }
-        } else if (tid == 0) {
-            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-            // relaxed polls (an acquire per poll would invalidate the caches other
-            // workgroups read from); the acquire fence after the barrier below follows
-            while (__hip_atomic_load(&P.sync[2 + 2 * inst], __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) != target) {
-                __builtin_amdgcn_s_sleep(2);
-                if (__builtin_amdgcn_s_memrealtime() - t0 > kPersistWaitTicks) {
-                    __hip_atomic_store(&P.sync[0], 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    red_s[101] = 1;
-                    break;
-                }
-            }
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (red_s[101]) break;  // gave up waiting: the host reports the fault
-        const InstState stn = P.state[inst];
-        if (!stn.active) break;  // limit or goal reached at this step
-        pos = P.pos[g];
+    if (iv > 0) {  // ---- exchange step between intervals (persistent launch) --------
+        xstat = persistent_exchange(P, d, inst, g, local, E, bestE, gen0 + (uint32_t)iv, mirror,
+                                    xm, red_s, tid, &pos, &sweep0);
+        if (xstat != kXmRun) break;  // the instance ends here, or a wait gave up
         beta2f = P.beta2f[d.beta_off + pos];
         c20 = P.c20[d.beta_off + pos];
-        sweep0 = stn.sweep_base;
     }
     for (int s = 0; s < d.spe; ++s) {
         const uint32_t sweep = sweep0 + (uint32_t)s;
@@ -925,7 +959,17 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         }
         P.cells[g] += tot;
     }
+    // the mirrored exchange state, left as exchange_instance would have left it
+    if (mirror && local == 0 && wid == 0 && xstat != kXmFault)
+        xm_store(P, inst, d, xm, xstat == kXmRun, lane);
 }
+
+}  // namespace qap
+
+// the speculative sweep kernel shares this file's exchange step and persistent protocol
+#include "qap_pt_sweep_spec.h"
+
+namespace qap {
 
 // ---------------------------------------------------------------------------
 // Exchange: one workgroup per instance.  Energies, bests, ladder order and positions
@@ -935,22 +979,6 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
 // ---------------------------------------------------------------------------
 constexpr int kExMaxRT = 2048;
 constexpr int kExWaveLadders = 8;  // at most this many ladders of <= 64 rungs: wave path
-
-__device__ __forceinline__ int64_t readlane64(int64_t v, int lane) {
-    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, lane);
-    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), lane);
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-// minimum over the wave, every lane gets it
-__device__ __forceinline__ int64_t wave_min64(int64_t v) {
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int64_t o = __shfl_xor(v, off);
-        v = o < v ? o : v;
-    }
-    return v;
-}
 
 // LDS of an exchange workgroup staging up to cap replicas: sized by the pool, so a launch
 // does not ask for room for kExMaxRT replicas beside the sweep workgroups of other pools
@@ -1183,8 +1211,46 @@ static hipError_t launch_sweep_c(const PoolDev &P, const SweepShape &sh, const i
     }
 }
 
+// The speculative kernel, launched like launch_sweep_t (max_iv 0: occupancy check only).
+template <int S>
+static hipError_t launch_spec_t(const PoolDev &P, const int32_t *list, int nblocks, int lds_bytes,
+                                int max_iv, hipStream_t s) {
+    auto k = sweep_spec_kernel<S>;
+    const hipError_t e = allow_full_lds<sweep_spec_kernel<S>>();
+    if (e != hipSuccess) return e;
+    if (max_iv == 0) {
+        int per_cu = 0, dev = 0, cus = 0;
+        hipError_t q = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(k), S * kWave, (size_t)lds_bytes);
+        if (q == hipSuccess) q = hipGetDevice(&dev);
+        if (q == hipSuccess) q = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (q != hipSuccess) return q;
+        return (int64_t)per_cu * cus >= nblocks ? hipSuccess : hipErrorCooperativeLaunchTooLarge;
+    }
+    if (max_iv <= 1) {
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(S * kWave), lds_bytes, s, P, list, 1);
+        return hipGetLastError();
+    }
+    PoolDev pd = P;
+    const int32_t *lp = list;
+    int iv = max_iv;
+    void *args[] = {&pd, &lp, &iv};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k), dim3(nblocks),
+                                      dim3(S * kWave), args, (unsigned)lds_bytes, s);
+}
+
 static hipError_t launch_sweep_iv(const PoolDev &P, const SweepShape &sh, const int32_t *list,
                                   int nblocks, int lds_bytes, int max_iv, hipStream_t s) {
+    if (sh.spec > 0) {
+        if (sh.im != IM_LDS32 || !sh.sym) return hipErrorInvalidValue;
+        switch (sh.spec) {
+        case 2: return launch_spec_t<2>(P, list, nblocks, lds_bytes, max_iv, s);
+        case 4: return launch_spec_t<4>(P, list, nblocks, lds_bytes, max_iv, s);
+        case 8: return launch_spec_t<8>(P, list, nblocks, lds_bytes, max_iv, s);
+        case 16: return launch_spec_t<16>(P, list, nblocks, lds_bytes, max_iv, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (sh.nt * 8 + sh.cpl) {
     case 64 * 8 + 1: return launch_sweep_c<64, 1>(P, sh, list, nblocks, lds_bytes, max_iv, s);
     case 64 * 8 + 2: return launch_sweep_c<64, 2>(P, sh, list, nblocks, lds_bytes, max_iv, s);
@@ -1229,6 +1295,11 @@ bool sweep_shape_supported(int nt, int cpl, int im) {
     case 256 * 8 + 1: case 256 * 8 + 2: case 256 * 8 + 4: return true;
     default: return false;
     }
+}
+
+int shape_lds_bytes(int n, int ld_s, const SweepShape &sh) {
+    if (sh.spec > 0) return 4 * spec_lds_words(n, ld_s, sh.spec);
+    return sweep_lds_bytes(n, ld_s, sh.im);
 }
 
 int sweep_lds_bytes(int n, int ld_s, int im) {

@@ -282,6 +282,42 @@ CASES = {
     "grey_c2_n256_R64": (lambda q: q.instances.grey_c2(), 64, 3,
                          {"ladder": (10.0, 1000.0),
                           "expect_shape": "256x1,lds16,sym,sparse-flow"}),
+    # several sites per round (sweep_spec_kernel, the latency-bound default for symmetric
+    # n <= 64 in int32 LDS) and one site at a time, on mostly rejected updates (the n = 64
+    # flat instance on T 1..100) and on mostly accepted ones (a hot ladder)
+    "c1_n64_R16_spec8": (lambda q: q.instances.flat_c1(), 16, 200,
+                         {"ladder": (1.0, 100.0), "expect_shape": "64x1,lds32,sym,spec8"}),
+    "c1_n64_R16_spec4": (lambda q: q.instances.flat_c1(), 16, 100,
+                         {"ladder": (1.0, 100.0), "env": {"QAP_PT_SPEC": "4"},
+                          "expect_shape": "64x1,lds32,sym,spec4"}),
+    "c1_n64_R16_spec2": (lambda q: q.instances.flat_c1(), 16, 100,
+                         {"ladder": (1.0, 100.0), "env": {"QAP_PT_SPEC": "2"},
+                          "expect_shape": "64x1,lds32,sym,spec2"}),
+    "c1_n64_R16_one_site": (lambda q: q.instances.flat_c1(), 16, 100,
+                            {"ladder": (1.0, 100.0), "env": {"QAP_PT_SPEC": "0"},
+                             "expect_shape": "64x1,lds32,sym,dense"}),
+    "spec8_hot_n50": (lambda q: q.instances.random_instance(80, 50), 8, 30,
+                      {"beta_scale": 0.01, "expect_shape": "64x1,lds32,sym,spec8"}),
+    "spec8_n2": (lambda q: q.instances.random_instance(81, 2), 4, 20,
+                 {"expect_shape": "64x1,lds32,sym,spec8"}),
+    "spec4_n2": (lambda q: q.instances.random_instance(81, 2), 4, 20,
+                 {"env": {"QAP_PT_SPEC": "4"}, "expect_shape": "64x1,lds32,sym,spec4"}),
+    "spec2_n1": (lambda q: q.instances.random_instance(82, 1), 3, 5,
+                 {"env": {"QAP_PT_SPEC": "2"}}),
+    "spec4_n63_T3_kick": (lambda q: q.instances.random_instance(83, 63), 5, 40,
+                          {"T": 3, "kick_after": 2, "env": {"QAP_PT_SPEC": "4"}}),
+    "spec8_n63_T3_kick": (lambda q: q.instances.random_instance(83, 63), 5, 40,
+                          {"T": 3, "kick_after": 2}),
+    "spec8_hot_n33_kick": (lambda q: q.instances.random_instance(85, 33), 6, 30,
+                           {"beta_scale": 0.01, "kick_after": 2,
+                            "expect_shape": "64x1,lds32,sym,spec8"}),
+    "spec8_n5": (lambda q: q.instances.random_instance(86, 5), 4, 20, {}),
+    "c1_n64_R16_spec16": (lambda q: q.instances.flat_c1(), 16, 100,
+                          {"ladder": (1.0, 100.0), "env": {"QAP_PT_SPEC": "16"},
+                           "expect_shape": "64x1,lds32,sym,spec16"}),
+    "spec16_hot_n21_kick": (lambda q: q.instances.random_instance(88, 21), 5, 30,
+                            {"beta_scale": 0.01, "kick_after": 2, "env": {"QAP_PT_SPEC": "16"},
+                             "expect_shape": "64x1,lds32,sym,spec16"}),
 }
 
 
@@ -299,9 +335,20 @@ def test_trajectory_bit_exact(qpt, oracle, gpu, name):
     else:  # the workload's own ladder (temperatures t_min .. t_max)
         betas = qpt.instances.geometric_ladder(R, *ladder)
     print(f"case {name}: n={n} R={R} sweeps={sweeps} kw={kw}", flush=True)
-    if expect_shape is not None:
-        with qpt.Pool([qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=0)]) as pool:
-            assert pool.shape(0).startswith(expect_shape), pool.shape(0)
+    if expect_shape is not None:  # under the case's environment, as the run below
+        env = kw.get("env", {})
+        old = {k: os.environ.get(k) for k in env}
+        os.environ.update(env)
+        try:
+            with qpt.Pool([qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=0)]) as pool:
+                shape = pool.shape(0)
+        finally:
+            for k, v in old.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        assert shape.startswith(expect_shape), shape
     res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, sweeps, **kw)
     _assert_same(res, st, ver, ref)
 
@@ -444,33 +491,70 @@ PERSIST_CASES = {
                            {"T": 3, "kick_after": 2}),
     "n100_256x1_T12": (lambda q: q.instances.mixed_instance(5, 100, True), 4, 10,
                        {"T": 12, "env": {"QAP_PT_SHAPE": "256,1"}}),
+    "spec4_n40_T5": (lambda q: q.instances.random_instance(84, 40), 6, 30,
+                     {"T": 5, "env": {"QAP_PT_SPEC": "4"}}),
+    "spec8_n40_T5": (lambda q: q.instances.random_instance(87, 40), 6, 30, {"T": 5}),
 }
 
 
 @pytest.mark.parametrize("name", list(PERSIST_CASES))
 def test_persistent_launch_equals_interval_launches(qpt, oracle, gpu, name):
-    """The exchange step inside a persistent launch (last replica of the instance to arrive)
-    gives the same trajectory as exchange launches between sweep launches, and both equal
-    the oracle's; a goal stops both at the same interval."""
+    """The exchange step inside a persistent launch -- mirrored by every workgroup from the
+    replicas' tagged costs (ladders of up to 64 replicas), or run by the last replica of the
+    instance to arrive (QAP_PT_XMIRROR=0, and every larger ladder set) -- gives the same
+    trajectory as exchange launches between sweep launches, and all equal the oracle's; a
+    goal stops each at the same interval."""
     build, R, sweeps, kw = PERSIST_CASES[name]
     F, D = build(qpt)
     n = F.shape[0]
     betas = qpt.instances.geometric_ladder(R, 1.0, 100.0) / max(1.0, n / 16.0)
     print(f"persist case {name}: n={n} R={R} sweeps={sweeps} kw={kw}", flush=True)
     env = dict(kw.pop("env", {}))
-    out = {}
-    for persist in ("1", "0"):
+    modes = {"mirror": {"QAP_PT_PERSIST": "1"},
+             "counter": {"QAP_PT_PERSIST": "1", "QAP_PT_XMIRROR": "0"},
+             "launches": {"QAP_PT_PERSIST": "0"}}
+    for mode, menv in modes.items():
         res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, sweeps,
-                                      env=dict(env, QAP_PT_PERSIST=persist), **kw)
+                                      env=dict(env, **menv), **kw)
         _assert_same(res, st, ver, ref)
-        out[persist] = (res, st)
     free = oracle.solve(F, D, betas, sweeps, seed=7, **kw)
     goal = int(free["best_cost"] * 1.01)
-    for persist in ("1", "0"):
+    for mode, menv in modes.items():
         res, st, ver, ref = _run_both(qpt, oracle, F, D, betas, sweeps, goal=goal,
-                                      env=dict(env, QAP_PT_PERSIST=persist), **kw)
-        assert res["sweeps_done"] == ref["megas_done"] * kw.get("spe", 1)
+                                      env=dict(env, **menv), **kw)
+        assert res["sweeps_done"] == ref["megas_done"] * kw.get("spe", 1), mode
         _assert_same(res, st, ver, ref)
+
+
+@pytest.mark.parametrize("xmirror", ["1", "0"])
+def test_persistent_runs_split_across_launches(qpt, oracle, gpu, xmirror):
+    """A run advanced in uneven pieces (persistent launches of 1, 2, 7 and the remaining
+    intervals, each after an exchange launch) carries the ladder state from launch to
+    launch: the mirrored step's copy is written back at the end of each launch as the
+    exchange kernel would have left it.  Two ladders, kicks after two quiet steps."""
+    F, D = qpt.instances.random_instance(89, 30)
+    R, T, sweeps = 6, 2, 40
+    betas = qpt.instances.geometric_ladder(R, 1.0, 100.0) / 2.0
+    env = {"QAP_PT_PERSIST": "1", "QAP_PT_XMIRROR": xmirror}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        inst = qpt.Instance(F=F, D=D, beta_ln=betas, sweeps=sweeps, T=T, seed=11, kick_after=2)
+        with qpt.Pool([inst]) as pool:
+            for piece in (1, 2, 7, 5):
+                pool.run(piece)
+            pool.run(-1)
+            res, st = pool.result(0), pool.state(0)
+            ver = pool.verify()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    ref = oracle.solve(F, D, betas, sweeps, T=T, seed=11, kick_after=2)
+    assert ref["kicks"] > 0
+    _assert_same(res, st, ver, ref)
 
 
 def test_frozen_ladder_all_weights_zero(qpt, oracle, gpu):
