@@ -461,6 +461,31 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
     // facility at the current site, carried in a register from update to update (the
     // swap at site i moves it only when the partner location is the next site)
     int a_reg = a_cur;
+    // Rows are loaded two updates ahead of their use: during the update of site i the
+    // registers below receive distance row i+2 and the flow row of p(i+2); they are stored
+    // into the other LDS buffer during the update of site i+1, so an L2 round trip has a
+    // whole update to land instead of the part after the loads' issue.  a_rec is the
+    // facility whose flow row is in the registers (p(i+2) when issued: a swap in between
+    // that moves it is caught by the a != a_cur test at the top of the update).
+    int32_t dn[CPL], dtn[CPL], frn[CPL], ftrn[CPL];
+    int a_rec;
+    {
+        const int i1 = n > 1 ? 1 : 0;
+        a_rec = p_s[i1];
+#pragma unroll
+        for (int q = 0; q < CPL; ++q) {
+            const int bc = min(tid + q * NT, n - 1);
+            dn[q] = D[(int64_t)i1 * n + bc];
+            frn[q] = F[(int64_t)a_rec * n + bc];
+            if constexpr (!SYM) {
+                dtn[q] = Dt[(int64_t)i1 * n + bc];
+                ftrn[q] = Ft[(int64_t)a_rec * n + bc];
+            } else {
+                dtn[q] = 0;
+                ftrn[q] = 0;
+            }
+        }
+    }
 
     for (int iv = 0; iv < max_iv; ++iv) {
     if (iv > 0) {  // ---- exchange step between intervals (persistent launch) --------
@@ -520,20 +545,40 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
                     lmax = max(lmax, v);
                 }
             }
-            // prefetch for the next update: distance row i+1 and flow row p(i+1).  The
-            // loads feed LDS stores at the end of this update only, so nothing in
-            // between waits for them.
+            // rows of the next site (loaded during the previous update) into the other
+            // buffers, which nobody reads this update; then the loads for the site after
             const int inext = (i + 1 == n) ? 0 : i + 1;
+            const int inext2 = (inext + 1 == n) ? 0 : inext + 1;
             const int a_nx = p_s[inext];
-            int32_t dn[CPL], dtn[CPL], frn[CPL], ftrn[CPL];
+            const int a_nx2 = p_s[inext2];
+            {
+                int32_t *dnb = drow_s + (buf ^ 1) * n4;
+                int32_t *dtb = dcol_s + (buf ^ 1) * n4;
+                int32_t *fnb = frow_s + (buf ^ 1) * n4;
+                int32_t *ftb = ftrow_s + (buf ^ 1) * n4;
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) {
+                    const int bb = tid + q * NT;
+                    if (bb < n) {
+                        dnb[bb] = dn[q];
+                        fnb[bb] = frn[q];
+                        if constexpr (!SYM) {
+                            dtb[bb] = dtn[q];
+                            ftb[bb] = ftrn[q];
+                        }
+                    }
+                }
+            }
+            const int a_nbuf = a_rec;  // the facility whose flow row is now in buf ^ 1
+            a_rec = a_nx2;
 #pragma unroll
             for (int q = 0; q < CPL; ++q) {
                 const int bc = min(tid + q * NT, n - 1);
-                dn[q] = D[(int64_t)inext * n + bc];
-                frn[q] = F[(int64_t)a_nx * n + bc];
+                dn[q] = D[(int64_t)inext2 * n + bc];
+                frn[q] = F[(int64_t)a_nx2 * n + bc];
                 if constexpr (!SYM) {
-                    dtn[q] = Dt[(int64_t)inext * n + bc];
-                    ftrn[q] = Ft[(int64_t)a_nx * n + bc];
+                    dtn[q] = Dt[(int64_t)inext2 * n + bc];
+                    ftrn[q] = Ft[(int64_t)a_nx2 * n + bc];
                 }
             }
             int maxd;
@@ -606,25 +651,6 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
             if (owner && my_b != a) {  // the winner commits the permutation swap
                 p_s[i] = my_b;
                 p_s[my_j] = a;
-            }
-            // next rows into the other buffers (nobody reads them this update)
-            {
-                int32_t *dnb = drow_s + (buf ^ 1) * n4;
-                int32_t *dtb = dcol_s + (buf ^ 1) * n4;
-                int32_t *fnb = frow_s + (buf ^ 1) * n4;
-                int32_t *ftb = ftrow_s + (buf ^ 1) * n4;
-#pragma unroll
-                for (int q = 0; q < CPL; ++q) {
-                    const int bb = tid + q * NT;
-                    if (bb < n) {
-                        dnb[bb] = dn[q];
-                        fnb[bb] = frn[q];
-                        if constexpr (!SYM) {
-                            dtb[bb] = dtn[q];
-                            ftb[bb] = ftrn[q];
-                        }
-                    }
-                }
             }
             int b, j;
             int32_t dsel;
@@ -910,7 +936,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
             }
             // the prefetched rows become current
             buf ^= 1;
-            a_cur = a_nx;
+            a_cur = a_nbuf;
             a_reg = (b != a && j == inext) ? a : a_nx;
         }
     }

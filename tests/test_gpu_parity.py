@@ -206,6 +206,17 @@ CASES = {
     "twins_n48": (lambda q: _twins(60, 48), 4, 12, {}),
     "twins_n48_hbm": (lambda q: _twins(61, 48), 4, 12, {"env": {"QAP_PT_IMODE": "2"}}),
     "twins_n48_lds16": (lambda q: _twins(62, 48), 4, 12, {"env": {"QAP_PT_IMODE": "1"}}),
+    # twin updates of the uint16 placement land one update late (the next site's deltas
+    # carry their terms): longer runs, the one-wave shape, and partner sites next in line
+    "twins_n48_lds16_40sw": (lambda q: _twins(66, 48), 4, 40, {"env": {"QAP_PT_IMODE": "1"}}),
+    "twins_n48_64x4_lds16": (lambda q: _twins(67, 48), 4, 30,
+                             {"env": {"QAP_PT_IMODE": "1", "QAP_PT_SHAPE": "64,4"}}),
+    "twins_n9_lds16_kick": (lambda q: _twins(68, 9, blocks=((0, 5, 3),), pairs=()), 5, 60,
+                            {"kick_after": 2, "env": {"QAP_PT_IMODE": "1"}}),
+    "grey_n256_R8_20sw": (lambda q: q.instances.grey_c2(), 8, 20, {}),
+    "grey_n256_R4_64x4": (lambda q: q.instances.grey_c2(), 4, 4, {"env": {"QAP_PT_SHAPE": "64,4"}}),
+    "grey_n256_R4_128x2": (lambda q: q.instances.grey_c2(), 4, 4,
+                           {"env": {"QAP_PT_SHAPE": "128,2"}}),
     "twins_n48_64x1_T2_lds16": (lambda q: _twins(63, 48), 3, 12,
                                 {"T": 2, "env": {"QAP_PT_SHAPE": "64,1", "QAP_PT_IMODE": "1"}}),
     "twins_n150_128x2_lds16": (lambda q: _twins(64, 150, blocks=((0, 40, 2), (40, 60, 0)),
