@@ -22,9 +22,13 @@ namespace qap {
 
 constexpr int kSpecRes = 4;  // words per wave in the round's result slots: b, j, delta, a
 
+// The result slots are double-buffered by round parity: a round that commits nothing ends
+// without a barrier, and a wave that starts the next round may write its slot while another
+// still reads this round's; it writes that buffer again only after the next round's
+// barrier, which every wave reaches after its reads.
 __host__ __device__ inline int spec_lds_words(int n, int ld_s, int S) {
     const int n4 = round8(n);
-    return 6 * n4 + kSpecRes * S + kRedWords + 2 * n * n + n * ld_s;
+    return 6 * n4 + 2 * kSpecRes * S + kRedWords + 2 * n * n + n * ld_s;
 }
 
 template <int S>
@@ -48,8 +52,8 @@ __global__ __launch_bounds__(S * kWave) void sweep_spec_kernel(PoolDev P,
     uint32_t *rnd_s = reinterpret_cast<uint32_t *>(cls_s + n4);
     int32_t *u_s = reinterpret_cast<int32_t *>(rnd_s + n4);  // the committed move's u and v
     int32_t *v_s = u_s + n4;
-    int32_t *res_s = v_s + n4;                               // [S][kSpecRes]
-    int32_t *red_s = res_s + kSpecRes * S;
+    int32_t *res_s = v_s + n4;                               // [2][S][kSpecRes]
+    int32_t *red_s = res_s + 2 * kSpecRes * S;
     int32_t *D_s = red_s + kRedWords;  // D and F, n x n row-major
     int32_t *F_s = D_s + n * n;
     int32_t *I_s = F_s + n * n;        // force matrix, row stride ld_s
@@ -112,7 +116,8 @@ __global__ __launch_bounds__(S * kWave) void sweep_spec_kernel(PoolDev P,
                                .x;
             lds_sync<S>();
 
-            for (int i0 = 0; i0 < n;) {
+            for (int i0 = 0, par = 0; i0 < n; par ^= 1) {
+                int32_t *rs_s = res_s + par * (kSpecRes * S);  // this round's slots
                 // (1)-(3) wave w: site i0 + w against the current state (one-wave pick)
                 const int i = i0 + wid;
                 if (i < n) {  // wave-uniform
@@ -139,26 +144,28 @@ __global__ __launch_bounds__(S * kWave) void sweep_spec_kernel(PoolDev P,
                         dw = __builtin_amdgcn_readlane(pk.d, src);
                     }
                     if (lane == 0) {
-                        res_s[kSpecRes * wid + 0] = bw;
-                        res_s[kSpecRes * wid + 1] = jw;
-                        res_s[kSpecRes * wid + 2] = dw;
-                        res_s[kSpecRes * wid + 3] = a;
+                        rs_s[kSpecRes * wid + 0] = bw;
+                        rs_s[kSpecRes * wid + 1] = jw;
+                        rs_s[kSpecRes * wid + 2] = dw;
+                        rs_s[kSpecRes * wid + 3] = a;
                     }
                 }
                 lds_sync<S>();
-                // the first of this round's sites whose pick moved a facility
-                int ws = -1;
-#pragma unroll
-                for (int w = 0; w < S; ++w)
-                    if (ws < 0 && i0 + w < n && res_s[kSpecRes * w] != res_s[kSpecRes * w + 3]) ws = w;
-                if (ws < 0) {  // every site of the round stayed
+                // the first of this round's sites whose pick moved a facility: lane w reads
+                // site i0+w's slot (one 16-byte LDS read), a ballot finds the first move
+                v4i rs = {0, 0, 0, 0};
+                if (lane < S) rs = reinterpret_cast<const v4i *>(rs_s)[lane];
+                const uint64_t moved = __ballot(lane < S && i0 + lane < n && rs.x != rs.w);
+                if (moved == 0) {  // every site of the round stayed
                     i0 += S;
                     continue;
                 }
+                const int ws = __ffsll((unsigned long long)moved) - 1;
                 const int ic = i0 + ws;
-                const int b = res_s[kSpecRes * ws + 0], j = res_s[kSpecRes * ws + 1];
-                const int32_t dsel = res_s[kSpecRes * ws + 2];
-                const int a = res_s[kSpecRes * ws + 3];
+                const int b = __builtin_amdgcn_readlane(rs.x, ws);
+                const int j = __builtin_amdgcn_readlane(rs.y, ws);
+                const int32_t dsel = __builtin_amdgcn_readlane(rs.z, ws);
+                const int a = __builtin_amdgcn_readlane(rs.w, ws);
                 // (4) commit (CPU_PIM.cpp:252-283)
                 E -= dsel;
                 ++acc;

@@ -293,11 +293,14 @@ CASES = {
     "grey_c2_n256_R64": (lambda q: q.instances.grey_c2(), 64, 3,
                          {"ladder": (10.0, 1000.0),
                           "expect_shape": "256x1,lds16,sym,sparse-flow"}),
-    # several sites per round (sweep_spec_kernel, the latency-bound default for symmetric
-    # n <= 64 in int32 LDS) and one site at a time, on mostly rejected updates (the n = 64
-    # flat instance on T 1..100) and on mostly accepted ones (a hot ladder)
-    "c1_n64_R16_spec8": (lambda q: q.instances.flat_c1(), 16, 200,
-                         {"ladder": (1.0, 100.0), "expect_shape": "64x1,lds32,sym,spec8"}),
+    # several sites per round (sweep_spec_kernel, 16 by default, the latency-bound default
+    # for symmetric n <= 64 in int32 LDS) and one site at a time, on mostly rejected updates
+    # (the n = 64 flat instance on T 1..100) and on mostly accepted ones (a hot ladder)
+    "c1_n64_R16_spec16": (lambda q: q.instances.flat_c1(), 16, 200,
+                          {"ladder": (1.0, 100.0), "expect_shape": "64x1,lds32,sym,spec16"}),
+    "c1_n64_R16_spec8": (lambda q: q.instances.flat_c1(), 16, 100,
+                         {"ladder": (1.0, 100.0), "env": {"QAP_PT_SPEC": "8"},
+                          "expect_shape": "64x1,lds32,sym,spec8"}),
     "c1_n64_R16_spec4": (lambda q: q.instances.flat_c1(), 16, 100,
                          {"ladder": (1.0, 100.0), "env": {"QAP_PT_SPEC": "4"},
                           "expect_shape": "64x1,lds32,sym,spec4"}),
@@ -307,10 +310,13 @@ CASES = {
     "c1_n64_R16_one_site": (lambda q: q.instances.flat_c1(), 16, 100,
                             {"ladder": (1.0, 100.0), "env": {"QAP_PT_SPEC": "0"},
                              "expect_shape": "64x1,lds32,sym,dense"}),
+    "spec16_hot_n50": (lambda q: q.instances.random_instance(80, 50), 8, 30,
+                       {"beta_scale": 0.01, "expect_shape": "64x1,lds32,sym,spec16"}),
     "spec8_hot_n50": (lambda q: q.instances.random_instance(80, 50), 8, 30,
-                      {"beta_scale": 0.01, "expect_shape": "64x1,lds32,sym,spec8"}),
-    "spec8_n2": (lambda q: q.instances.random_instance(81, 2), 4, 20,
-                 {"expect_shape": "64x1,lds32,sym,spec8"}),
+                      {"beta_scale": 0.01, "env": {"QAP_PT_SPEC": "8"},
+                       "expect_shape": "64x1,lds32,sym,spec8"}),
+    "spec16_n2": (lambda q: q.instances.random_instance(81, 2), 4, 20,
+                  {"expect_shape": "64x1,lds32,sym,spec16"}),
     "spec4_n2": (lambda q: q.instances.random_instance(81, 2), 4, 20,
                  {"env": {"QAP_PT_SPEC": "4"}, "expect_shape": "64x1,lds32,sym,spec4"}),
     "spec2_n1": (lambda q: q.instances.random_instance(82, 1), 3, 5,
@@ -318,16 +324,15 @@ CASES = {
     "spec4_n63_T3_kick": (lambda q: q.instances.random_instance(83, 63), 5, 40,
                           {"T": 3, "kick_after": 2, "env": {"QAP_PT_SPEC": "4"}}),
     "spec8_n63_T3_kick": (lambda q: q.instances.random_instance(83, 63), 5, 40,
-                          {"T": 3, "kick_after": 2}),
+                          {"T": 3, "kick_after": 2, "env": {"QAP_PT_SPEC": "8"}}),
+    "spec16_n63_T3_kick": (lambda q: q.instances.random_instance(83, 63), 5, 40,
+                           {"T": 3, "kick_after": 2}),
     "spec8_hot_n33_kick": (lambda q: q.instances.random_instance(85, 33), 6, 30,
-                           {"beta_scale": 0.01, "kick_after": 2,
+                           {"beta_scale": 0.01, "kick_after": 2, "env": {"QAP_PT_SPEC": "8"},
                             "expect_shape": "64x1,lds32,sym,spec8"}),
-    "spec8_n5": (lambda q: q.instances.random_instance(86, 5), 4, 20, {}),
-    "c1_n64_R16_spec16": (lambda q: q.instances.flat_c1(), 16, 100,
-                          {"ladder": (1.0, 100.0), "env": {"QAP_PT_SPEC": "16"},
-                           "expect_shape": "64x1,lds32,sym,spec16"}),
+    "spec16_n5": (lambda q: q.instances.random_instance(86, 5), 4, 20, {}),
     "spec16_hot_n21_kick": (lambda q: q.instances.random_instance(88, 21), 5, 30,
-                            {"beta_scale": 0.01, "kick_after": 2, "env": {"QAP_PT_SPEC": "16"},
+                            {"beta_scale": 0.01, "kick_after": 2,
                              "expect_shape": "64x1,lds32,sym,spec16"}),
 }
 
@@ -504,7 +509,9 @@ PERSIST_CASES = {
                        {"T": 12, "env": {"QAP_PT_SHAPE": "256,1"}}),
     "spec4_n40_T5": (lambda q: q.instances.random_instance(84, 40), 6, 30,
                      {"T": 5, "env": {"QAP_PT_SPEC": "4"}}),
-    "spec8_n40_T5": (lambda q: q.instances.random_instance(87, 40), 6, 30, {"T": 5}),
+    "spec8_n40_T5": (lambda q: q.instances.random_instance(87, 40), 6, 30,
+                     {"T": 5, "env": {"QAP_PT_SPEC": "8"}}),
+    "spec16_n40_T5": (lambda q: q.instances.random_instance(90, 40), 6, 30, {"T": 5}),
 }
 
 
