@@ -492,6 +492,8 @@ __global__ __launch_bounds__(NT) void sweep_kernel(PoolDev P, const int32_t *__r
         xstat = persistent_exchange(P, d, inst, g, local, E, bestE, gen0 + (uint32_t)iv, mirror,
                                     xm, red_s, tid, &pos, &sweep0);
         if (xstat != kXmRun) break;  // the instance ends here, or a wait gave up
+        // (the temperature tables are not kept in registers here as in sweep_spec_kernel:
+        // the three extra registers cost grey n = 256 4 %)
         beta2f = P.beta2f[d.beta_off + pos];
         c20 = P.c20[d.beta_off + pos];
     }

@@ -91,6 +91,8 @@ __global__ __launch_bounds__(S * kWave) void sweep_spec_kernel(PoolDev P,
     XMirror xm{};
     int xstat = kXmRun;
     if (mirror && wid == 0) xm = xm_load(P, inst, d, lane);
+    XBeta bt{0.f, 0.0};
+    if (mirror) bt = xm_beta_load(P, d, lane);
     uint32_t gen0 = 0;
     if (max_iv > 1 && !mirror)
         gen0 = __hip_atomic_load(&P.sync[2 + 2 * inst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -104,8 +106,12 @@ __global__ __launch_bounds__(S * kWave) void sweep_spec_kernel(PoolDev P,
             xstat = persistent_exchange(P, d, inst, g, local, E, bestE, gen0 + (uint32_t)iv,
                                         mirror, xm, red_s, tid, &pos, &sweep0);
             if (xstat != kXmRun) break;
-            beta2f = P.beta2f[d.beta_off + pos];
-            c20 = P.c20[d.beta_off + pos];
+            if (mirror) {
+                xm_beta_at(bt, pos, beta2f, c20);
+            } else {
+                beta2f = P.beta2f[d.beta_off + pos];
+                c20 = P.c20[d.beta_off + pos];
+            }
         }
         for (int s = 0; s < d.spe; ++s) {
             const uint32_t sweep = sweep0 + (uint32_t)s;

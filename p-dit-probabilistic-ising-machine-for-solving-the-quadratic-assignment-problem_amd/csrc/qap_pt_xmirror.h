@@ -54,13 +54,32 @@ __device__ __forceinline__ uint64_t xm_word(uint32_t tag, int64_t v) {
     const int32_t lo = v >= (int64_t)INT32_MAX ? INT32_MAX : (int32_t)v;
     return ((uint64_t)tag << 32) | (uint32_t)lo;
 }
-__device__ __forceinline__ int64_t xm_value(uint64_t w) {
-    const int32_t lo = (int32_t)(uint32_t)w;
-    return lo == INT32_MAX ? INT64_MAX : (int64_t)lo;
-}
 // the two words of replica g for the step with exchange count m
 __device__ __forceinline__ uint64_t *xm_slot(const PoolDev &P, int g, int64_t m) {
     return P.xslot + 4 * (int64_t)g + 2 * (m & 1);
+}
+
+// Every wave's copy of the instance's per-position tables (lane l < R: position l), so that
+// a new position after a mirrored step reads its temperature from registers
+struct XBeta {
+    float beta2f;
+    double c20;
+};
+__device__ __forceinline__ XBeta xm_beta_load(const PoolDev &P, const InstDesc &d, int lane) {
+    XBeta t{0.f, 0.0};
+    if (lane < d.R) {
+        t.beta2f = P.beta2f[d.beta_off + lane];
+        t.c20 = P.c20[d.beta_off + lane];
+    }
+    return t;
+}
+__device__ __forceinline__ void xm_beta_at(const XBeta &t, int pos, float &beta2f, double &c20) {
+    const int p = __builtin_amdgcn_readfirstlane(pos);  // pos is the same in every lane
+    beta2f = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t.beta2f), p));
+    const uint64_t cb = (uint64_t)__double_as_longlong(t.c20);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)cb, p);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(cb >> 32), p);
+    c20 = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 // wave 0's copy of one instance's exchange state
@@ -156,29 +175,33 @@ __device__ int xm_exchange(const PoolDev &P, const InstDesc &d, XMirror &x, int 
         }
         __builtin_amdgcn_s_sleep(1);
     }
-    const int64_t e_rep = live ? xm_value(w0) : INT64_MAX;   // by replica
-    const int64_t b_rep = live ? xm_value(w1) : INT64_MAX;
+    // the words' low halves: costs (every one below INT32_MAX) and bests (INT32_MAX: none
+    // yet), compared as int32 -- the same order as the int64 values they stand for
+    const int32_t e_rep = live ? (int32_t)(uint32_t)w0 : INT32_MAX;  // by replica
+    const int32_t b_rep = live ? (int32_t)(uint32_t)w1 : INT32_MAX;
     // cost at each ladder position: lane t*R + k takes replica ord[t*R + k]'s
     const int lbase = live ? lane - lane % R : 0;
-    const int64_t e_pos = __shfl(e_rep, lbase + (live ? x.ordv : 0));
+    const int32_t e_pos = __shfl(e_rep, lbase + (live ? x.ordv : 0));
 
     int64_t gbest = INT64_MAX;
     uint32_t nex = 0, nkick = 0;
     for (int t = 0; t < T; ++t) {  // CPU_PIM.cpp:297-344, per ladder, as exchange_instance
         const int base = t * R;
         const bool inl = lane >= base && lane < base + R;
-        const int64_t mb = wave_min64(inl ? b_rep : INT64_MAX);
+        // minimum of the ladder's bests: a DPP maximum of the negated values
+        const int32_t mb32 = -lane63(wave_incl_max(inl ? -b_rep : INT32_MIN));
+        const int64_t mb = mb32 == INT32_MAX ? INT64_MAX : (int64_t)mb32;
         int32_t tss = __builtin_amdgcn_readlane(x.tssv, t) + 1;
         int64_t lbest = readlane64(x.lbestv, t);
         if (mb < lbest) {  // best fold
             lbest = mb;
             tss = 0;
         }
-        int64_t cE = readlane64(e_pos, base);
+        int32_t cE = __builtin_amdgcn_readlane(e_pos, base);
         int cR = __builtin_amdgcn_readlane(x.ordv, base);
         int newR = x.ordv;
         for (int k = 0; k < R - 1; ++k) {  // bubble pass
-            const int64_t nE = readlane64(e_pos, base + k + 1);
+            const int32_t nE = __builtin_amdgcn_readlane(e_pos, base + k + 1);
             const int nR = __builtin_amdgcn_readlane(x.ordv, base + k + 1);
             if (cE < nE) {
                 newR = lane == base + k ? nR : newR;
