@@ -28,7 +28,7 @@ constexpr int kSpecRes = 4;  // words per wave in the round's result slots: b, j
 // barrier, which every wave reaches after its reads.
 __host__ __device__ inline int spec_lds_words(int n, int ld_s, int S) {
     const int n4 = round8(n);
-    return 6 * n4 + 2 * kSpecRes * S + kRedWords + 2 * n * n + n * ld_s;
+    return 4 * n4 + 2 * kSpecRes * S + kRedWords + 2 * n * n + n * ld_s;
 }
 
 template <int S>
@@ -50,9 +50,7 @@ __global__ __launch_bounds__(S * kWave) void sweep_spec_kernel(PoolDev P,
     int32_t *pinv_s = p_s + n4;
     int32_t *cls_s = pinv_s + n4;
     uint32_t *rnd_s = reinterpret_cast<uint32_t *>(cls_s + n4);
-    int32_t *u_s = reinterpret_cast<int32_t *>(rnd_s + n4);  // the committed move's u and v
-    int32_t *v_s = u_s + n4;
-    int32_t *res_s = v_s + n4;                               // [2][S][kSpecRes]
+    int32_t *res_s = reinterpret_cast<int32_t *>(rnd_s + n4);  // [2][S][kSpecRes]
     int32_t *red_s = res_s + 2 * kSpecRes * S;
     int32_t *D_s = red_s + kRedWords;  // D and F, n x n row-major
     int32_t *F_s = D_s + n * n;
@@ -172,33 +170,29 @@ __global__ __launch_bounds__(S * kWave) void sweep_spec_kernel(PoolDev P,
                 const int j = __builtin_amdgcn_readlane(rs.y, ws);
                 const int32_t dsel = __builtin_amdgcn_readlane(rs.z, ws);
                 const int a = __builtin_amdgcn_readlane(rs.w, ws);
-                // (4) commit (CPU_PIM.cpp:252-283)
+                // (4) commit (CPU_PIM.cpp:252-283).  Every read of I and p this round came
+                // before the barrier above, and the next round's slots are the other buffer,
+                // so the commit writes without a barrier of its own.
                 E -= dsel;
                 ++acc;
                 if (lane == b) pv[0] = ic;
                 else if (lane == a) pv[0] = j;
-                const bool upd = ((cls_s[a] ^ cls_s[b]) & 0xffff) != 0;  // identical flow: v = 0
-                lds_sync<S>();  // every wave has read the round's results and the classes
                 if (tid == 0) {
                     p_s[ic] = b;
                     p_s[j] = a;
                 }
-                if (upd) {
-                    // u = D[j] - D[ic] (rows), v = 2 (F[b] - F[a]) (columns): I += u v^T
-                    uint32_t nz = 0;
-                    for (int x = tid; x < n; x += NT) {
-                        u_s[x] = D_s[j * n + x] - D_s[ic * n + x];
-                        const int32_t v = 2 * (F_s[b * n + x] - F_s[a * n + x]);
-                        v_s[x] = v;
-                        nz += v != 0;
+                if (((cls_s[a] ^ cls_s[b]) & 0xffff) != 0) {  // identical flow: v = 0
+                    // I += u v^T, u = D[j] - D[ic] (rows), v = 2 (F[b] - F[a]) (columns):
+                    // column c = lane, rows wid, wid + S, ...
+                    const int c = min(lane, n - 1);
+                    const int32_t vc = 2 * (F_s[b * n + c] - F_s[a * n + c]);
+                    if (wid == 0) {  // nonzero v columns (n <= 64: one wave holds every column)
+                        const uint64_t nzm = __ballot(lane < n && vc != 0);
+                        if (lane == 0) cells += (uint64_t)n * (uint64_t)__popcll(nzm);
                     }
-                    // nonzero v columns (n <= 64: the first wave holds every column)
-                    const uint32_t cnt = (uint32_t)__popcll(__ballot(nz != 0));
-                    if (tid == 0) cells += (uint64_t)n * cnt;
-                    lds_sync<S>();
-                    for (int e = tid; e < n * n; e += NT) {
-                        const int x = e / n, c = e - x * n;
-                        I_s[x * ld_s + c] += u_s[x] * v_s[c];
+                    for (int x = wid; x < n; x += S) {
+                        const int32_t ux = D_s[j * n + x] - D_s[ic * n + x];
+                        if (lane < n) I_s[x * ld_s + lane] += ux * vc;
                     }
                 }
                 lds_sync<S>();  // the updated matrix and permutation before the next round
