@@ -197,23 +197,42 @@ __device__ int xm_exchange(const PoolDev &P, const InstDesc &d, XMirror &x, int 
             lbest = mb;
             tss = 0;
         }
-        int32_t cE = __builtin_amdgcn_readlane(e_pos, base);
-        int cR = __builtin_amdgcn_readlane(x.ordv, base);
         int newR = x.ordv;
-        for (int k = 0; k < R - 1; ++k) {  // bubble pass
-            const int32_t nE = __builtin_amdgcn_readlane(e_pos, base + k + 1);
-            const int nR = __builtin_amdgcn_readlane(x.ordv, base + k + 1);
-            if (cE < nE) {
-                newR = lane == base + k ? nR : newR;
-                ++nex;
-                if (nR == R - 1) tss = 0;
-            } else {
-                newR = lane == base + k ? cR : newR;
-                cE = nE;
-                cR = nR;
+        if (T == 1) {
+            // The bubble pass in parallel, lane k = position k.  The sequential pass carries
+            // an element along the ladder and hands it the next one's place whenever the next
+            // costs at most as much, so the element carried into step k is the last position
+            // <= k whose cost equals the prefix minimum (a "record"); step k puts the next
+            // element at k when k + 1 is not a record (a swap: cE < nE), else the carry.
+            const int32_t ek = lane < R ? e_pos : INT32_MAX;
+            const int32_t pmin = -wave_incl_max(-ek);  // costs stay above INT32_MIN
+            const bool rec = lane < R && ek == pmin;
+            const int carry = wave_incl_max(rec ? lane : -1);
+            const int rcar = __shfl(x.ordv, carry);
+            const int rnext = __shfl(x.ordv, lane + 1);
+            const bool recn = __shfl((int)rec, lane + 1) != 0;
+            const bool swap = lane < R - 1 && !recn;
+            newR = lane < R - 1 ? (swap ? rnext : rcar) : (lane == R - 1 ? rcar : newR);
+            nex += (uint32_t)__popcll(__ballot(swap));
+            if (__ballot(swap && rnext == R - 1)) tss = 0;
+        } else {
+            int32_t cE = __builtin_amdgcn_readlane(e_pos, base);
+            int cR = __builtin_amdgcn_readlane(x.ordv, base);
+            for (int k = 0; k < R - 1; ++k) {  // bubble pass, step by step
+                const int32_t nE = __builtin_amdgcn_readlane(e_pos, base + k + 1);
+                const int nR = __builtin_amdgcn_readlane(x.ordv, base + k + 1);
+                if (cE < nE) {
+                    newR = lane == base + k ? nR : newR;
+                    ++nex;
+                    if (nR == R - 1) tss = 0;
+                } else {
+                    newR = lane == base + k ? cR : newR;
+                    cE = nE;
+                    cR = nR;
+                }
             }
+            newR = lane == base + R - 1 ? cR : newR;
         }
-        newR = lane == base + R - 1 ? cR : newR;
         if (tss > d.kick_after) {  // kick: the hottest and coldest positions trade
             const int r0 = __builtin_amdgcn_readlane(newR, base);
             const int rL = __builtin_amdgcn_readlane(newR, base + R - 1);
