@@ -293,6 +293,13 @@ CASES = {
     "grey_c2_n256_R64": (lambda q: q.instances.grey_c2(), 64, 3,
                          {"ladder": (10.0, 1000.0),
                           "expect_shape": "256x1,lds16,sym,sparse-flow"}),
+    # ... for 20 exchange steps in one persistent launch: the mirrored step's parallel
+    # bubble pass over a full wave of ladder positions
+    "grey_c2_n256_R64_20sw": (lambda q: q.instances.grey_c2(), 64, 20,
+                              {"ladder": (10.0, 1000.0),
+                               "expect_shape": "256x1,lds16,sym,sparse-flow"}),
+    # a ladder of one replica (the pass with no step)
+    "single_replica_n20": (lambda q: q.instances.random_instance(91, 20), 1, 30, {}),
     # several sites per round (sweep_spec_kernel, 16 by default, the latency-bound default
     # for symmetric n <= 64 in int32 LDS) and one site at a time, on mostly rejected updates
     # (the n = 64 flat instance on T 1..100) and on mostly accepted ones (a hot ladder)
