@@ -519,6 +519,9 @@ PERSIST_CASES = {
     "spec8_n40_T5": (lambda q: q.instances.random_instance(87, 40), 6, 30,
                      {"T": 5, "env": {"QAP_PT_SPEC": "8"}}),
     "spec16_n40_T5": (lambda q: q.instances.random_instance(90, 40), 6, 30, {"T": 5}),
+    # one ladder (the mirrored step's parallel bubble pass), three sweeps per exchange
+    "spec16_n40_R12_spe3_kick": (lambda q: q.instances.random_instance(92, 40), 12, 36,
+                                 {"spe": 3, "kick_after": 2}),
 }
 
 
