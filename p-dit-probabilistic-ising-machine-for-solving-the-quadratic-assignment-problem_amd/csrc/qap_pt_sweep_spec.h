@@ -99,8 +99,20 @@ __global__ __launch_bounds__(S * kWave) void sweep_spec_kernel(PoolDev P,
     int32_t pv[1];
     pv[0] = lane < n ? pinv_s[lane] : 0;
 
+    bool pre = false;  // rnd_s already holds the Philox words of sweep pre_sweep
+    uint32_t pre_sweep = 0;
     for (int iv = 0; iv < max_iv; ++iv) {
         if (iv > 0) {  // exchange step between intervals (persistent launch)
+            // while wave 0 runs a mirrored step, wave 1 draws the next interval's first sweep
+            // (every read of rnd_s is behind the last round's barrier; the step's barrier
+            // publishes the words)
+            pre_sweep = sweep0 + (uint32_t)d.spe;
+            pre = mirror && S > 1;
+            if (pre && wid == 1)
+                for (int x = lane; x < n; x += kWave)
+                    rnd_s[x] = philox4x32_10(u32x4{(uint32_t)x, pre_sweep, (uint32_t)local, 0u},
+                                             d.seed_lo, d.seed_hi)
+                                   .x;
             xstat = persistent_exchange(P, d, inst, g, local, E, bestE, gen0 + (uint32_t)iv,
                                         mirror, xm, red_s, tid, &pos, &sweep0);
             if (xstat != kXmRun) break;
@@ -113,12 +125,14 @@ __global__ __launch_bounds__(S * kWave) void sweep_spec_kernel(PoolDev P,
         }
         for (int s = 0; s < d.spe; ++s) {
             const uint32_t sweep = sweep0 + (uint32_t)s;
-            lds_sync<S>();
-            for (int x = tid; x < n; x += NT)
-                rnd_s[x] = philox4x32_10(u32x4{(uint32_t)x, sweep, (uint32_t)local, 0u}, d.seed_lo,
-                                         d.seed_hi)
-                               .x;
-            lds_sync<S>();
+            if (!(pre && s == 0 && sweep == pre_sweep)) {
+                lds_sync<S>();
+                for (int x = tid; x < n; x += NT)
+                    rnd_s[x] = philox4x32_10(u32x4{(uint32_t)x, sweep, (uint32_t)local, 0u},
+                                             d.seed_lo, d.seed_hi)
+                                   .x;
+                lds_sync<S>();
+            }
 
             for (int i0 = 0, par = 0; i0 < n; par ^= 1) {
                 int32_t *rs_s = res_s + par * (kSpecRes * S);  // this round's slots
