@@ -106,6 +106,15 @@ def _twins(seed, n, blocks=((0, 8, 4), (8, 16, 9), (16, 24, 0)), pairs=((30, 31,
     return F, (D + D.T).astype(np.int32)
 
 
+def _wide_costs(seed, n):
+    """Symmetric zero-diagonal flow in [0, 2^16) and distances in [0, 40]: the force bound
+    fits int32 but the cost bound max|F| sum|D| passes 2^31."""
+    g = np.random.default_rng(seed)
+    F = np.triu(g.integers(0, 1 << 16, (n, n)), 1)
+    D = np.triu(g.integers(0, 41, (n, n)), 1)
+    return (F + F.T).astype(np.int32), (D + D.T).astype(np.int32)
+
+
 def _wide64(seed, n, symmetric=True, zero_diag=True, big=1 << 27):
     """Flow entries up to 2^27 and distances up to 63: force bounds beyond int32, so the
     instance runs with int64 force matrices; costs reach ~1e13."""
@@ -519,6 +528,9 @@ PERSIST_CASES = {
     "spec8_n40_T5": (lambda q: q.instances.random_instance(87, 40), 6, 30,
                      {"T": 5, "env": {"QAP_PT_SPEC": "8"}}),
     "spec16_n40_T5": (lambda q: q.instances.random_instance(90, 40), 6, 30, {"T": 5}),
+    # costs beyond 31 bits by the host's bound (max|F| sum|D| ~ 6e9) while the force matrix
+    # still fits int32 LDS: the library keeps the counter protocol by itself
+    "wide_costs_n48_R16": (lambda q: _wide_costs(93, 48), 16, 30, {}),
     # one ladder (the mirrored step's parallel bubble pass), three sweeps per exchange
     "spec16_n40_R12_spe3_kick": (lambda q: q.instances.random_instance(92, 40), 12, 36,
                                  {"spe": 3, "kick_after": 2}),
