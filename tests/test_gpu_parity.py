@@ -567,6 +567,45 @@ def test_persistent_launch_equals_interval_launches(qpt, oracle, gpu, name):
 
 
 @pytest.mark.parametrize("xmirror", ["1", "0"])
+def test_persistent_launch_shared_by_instances_one_stopping(qpt, oracle, gpu, xmirror):
+    """Three instances of different n in one persistent launch (one kernel bucket), one of
+    them with a goal it reaches partway: it leaves the launch at that exchange step while
+    the others run on, each exchanging through its own replicas' words.  Every instance
+    equals its own oracle run."""
+    env = {"QAP_PT_PERSIST": "1", "QAP_PT_XMIRROR": xmirror}
+    sweeps, R = 40, 8
+    specs = [(94, 40), (95, 52), (96, 64)]
+    data = [qpt.instances.random_instance(sd, n) for sd, n in specs]
+    ladders = [qpt.instances.geometric_ladder(R, 1.0, 100.0) / max(1.0, n / 16.0)
+               for _, n in specs]
+    free = oracle.solve(*data[1], ladders[1], sweeps, seed=31)
+    goals = [None, int(free["best_cost"] * 1.02), None]
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        insts = [qpt.Instance(F=F, D=D, beta_ln=b, sweeps=sweeps, seed=30 + k, goal=gl)
+                 for k, ((F, D), b, gl) in enumerate(zip(data, ladders, goals))]
+        with qpt.Pool(insts) as pool:
+            shapes = {pool.shape(k) for k in range(3)}
+            pool.run(-1)
+            got = [(pool.result(k), pool.state(k)) for k in range(3)]
+            ver = pool.verify()
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    assert len(shapes) == 1, shapes  # one bucket: one launch holds all three
+    for k, ((F, D), b, gl) in enumerate(zip(data, ladders, goals)):
+        ref = oracle.solve(F, D, b, sweeps, seed=30 + k, goal=gl)
+        res, st = got[k]
+        _assert_same(res, st, ver, ref)
+        assert res["sweeps_done"] == ref["megas_done"], k
+    assert got[1][0]["sweeps_done"] < sweeps  # the goal ended instance 1 early
+
+
+@pytest.mark.parametrize("xmirror", ["1", "0"])
 def test_persistent_runs_split_across_launches(qpt, oracle, gpu, xmirror):
     """A run advanced in uneven pieces (persistent launches of 1, 2, 7 and the remaining
     intervals, each after an exchange launch) carries the ladder state from launch to
